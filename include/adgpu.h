@@ -1,0 +1,224 @@
+/*
+ * adgpu.h -- C ABI of the MI355X (gfx950) docking hot path.
+ *
+ * This is the drop-in boundary between AutoDock-GPU's host driver and the
+ * HIP kernels of this library.  Every entry point takes plain pointers and
+ * sizes, returns an adgpu_status code instead of calling exit(), and never
+ * exposes HIP or torch types.  Each declaration names the reference
+ * interface it replaces (file:line in ccsb-scripps/AutoDock-GPU v1.5).
+ *
+ * Units and layouts follow the reference device contract:
+ *   - genotypes are [x, y, z (grid units), phi, theta, alpha (deg),
+ *     torsion_1 .. torsion_T (deg)] padded to ADGPU_GENOTYPE_STRIDE floats
+ *     (GENOTYPE_LENGTH_IN_GLOBMEM, common/defines.h:94);
+ *   - populations are [run][individual][64], energies [run][individual]
+ *     (host/src/performdocking.cpp.Cuda:293-299);
+ *   - receptor maps use the packed-4 layout of host/src/processgrid.cpp:234-249:
+ *     each grid point of each map stores v(x,y,z), v(x,y+1,z), v(x,y,z+1),
+ *     v(x,y+1,z+1); maps are ordered [type maps..., electrostatic, desolvation].
+ */
+#ifndef ADGPU_H_
+#define ADGPU_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Limits (common/defines.h:83-95). */
+#define ADGPU_MAX_ATOMS          256
+#define ADGPU_MAX_ROTBONDS       57
+#define ADGPU_MAX_ATYPES         32
+#define ADGPU_MAX_PAIRS          (ADGPU_MAX_ATOMS * ADGPU_MAX_ATOMS)
+#define ADGPU_MAX_POPSIZE        2048
+#define ADGPU_MAX_RUNS           8192
+#define ADGPU_MAX_GRIDPOINTS     256
+#define ADGPU_GENOTYPE_STRIDE    64
+/* Number of PRNG streams per individual.  The reference keeps one LCG state
+ * per work-item (device/auxiliary_genetic.cl:29-51), i.e. NUM_OF_THREADS_PER_BLOCK
+ * states per block; this library always uses the 64-work-item layout. */
+#define ADGPU_NWI                64
+
+typedef enum adgpu_status {
+    ADGPU_OK            = 0,
+    ADGPU_ERR_ARG       = 1,   /* null pointer, bad id, inconsistent sizes     */
+    ADGPU_ERR_LIMIT     = 2,   /* a size exceeds one of the limits above        */
+    ADGPU_ERR_HIP       = 3,   /* a HIP runtime call failed                     */
+    ADGPU_ERR_NOMEM     = 4,   /* device memory could not be allocated          */
+    ADGPU_ERR_STATE     = 5    /* call not valid in the current object state    */
+} adgpu_status;
+
+typedef enum adgpu_map_format {
+    ADGPU_MAP_F32  = 0,
+    ADGPU_MAP_F16  = 1,
+    ADGPU_MAP_BF16 = 2
+} adgpu_map_format;
+
+typedef enum adgpu_ls_method {
+    ADGPU_LS_NONE  = 0,        /* plain GA (lsearch_rate == 0)                  */
+    ADGPU_LS_SW    = 1,        /* Solis-Wets, device/kernel3.cl                 */
+    ADGPU_LS_AD    = 2,        /* ADADELTA, device/kernel_ad.cl                 */
+    ADGPU_LS_ADAM  = 3,        /* ADAM, cuda/kernel_adam.cu                     */
+    ADGPU_LS_FIRE  = 4         /* FIRE, device/kernel_fire.cl                   */
+} adgpu_ls_method;
+
+typedef enum adgpu_eval_mode {
+    ADGPU_EVAL_ENERGY   = 0,   /* gpu_calc_energy   (device/calcenergy.cl:132)        */
+    ADGPU_EVAL_ENERGRAD = 1,   /* gpu_calc_energrad (device/calcMergedEneGra.cl:46)   */
+    ADGPU_EVAL_GRADIENT = 2    /* gpu_calc_gradient (device/calcgradient.cl:109)      */
+} adgpu_eval_mode;
+
+/* Receptor grid.  Mirrors the fields of Gridinfo the kernels read
+ * (host/inc/processgrid.h:43-57; dockpars gridsize/spacing at
+ * host/src/performdocking.cpp.Cuda:420-426). */
+typedef struct adgpu_grid {
+    int32_t      size_x, size_y, size_z;   /* grid points per axis (<= 256)      */
+    int32_t      num_map_atypes;           /* type maps; e and d follow them     */
+    float        spacing;                  /* Angstrom per grid unit             */
+    const float* packed;                   /* 4*size_x*size_y*size_z*(num_map_atypes+2) */
+} adgpu_grid;
+
+/* Ligand (+ flexible residue) device constants.  Mirrors the content of the
+ * kernelconstant_* blocks built by prepare_const_fields_for_gpu
+ * (host/src/calcenergy.cpp:31-345, host/inc/calcenergy.h:87-136) without the
+ * MAX_* padding.  Coordinates are already centred and in grid units, as
+ * gen_initpop_and_reflig leaves them (host/src/getparameters.cpp:2624-2630). */
+typedef struct adgpu_ligand {
+    int32_t         num_atoms;          /* ligand + flexres atoms (<= 256)        */
+    int32_t         true_ligand_atoms;  /* atoms 0..true_ligand_atoms-1 are ligand */
+    int32_t         num_rotbonds;       /* genes = num_rotbonds + 6 (<= 57)       */
+    int32_t         num_atypes;         /* ligand atom types (<= 32)              */
+    int32_t         num_pairs;          /* intramolecular contributors            */
+    const float*    charges;            /* [num_atoms]                            */
+    const int32_t*  atom_types;         /* [num_atoms] ligand type index          */
+    const int32_t*  atom_map;           /* [num_atoms] index of the type map      */
+    const uint8_t*  ignore_inter;       /* [num_atoms] 1 = no receptor energy     */
+    const float*    ref_coords;         /* [3*num_atoms]                          */
+    const float*    rotbond_moving;     /* [3*num_rotbonds] moving vectors        */
+    const float*    rotbond_unit;       /* [3*num_rotbonds] unit vectors          */
+    const int32_t*  rotbond_atoms;      /* [2*num_rotbonds] the bond's two atoms  */
+    const uint8_t*  atom_rotbonds;      /* [num_atoms*num_rotbonds] 1 = atom moves */
+    const int32_t*  pairs;              /* [2*num_pairs] (i<j)                    */
+    const uint32_t* atype_reqm;         /* [num_atypes] base type index           */
+    const uint16_t* vw_exp;             /* [num_atypes^2] (m<<8)|n                */
+    const float*    reqm_ab;            /* [num_atypes^2]                         */
+    const float*    vw_ac;              /* [num_atypes^2] A (vdW) or C (H-bond)   */
+    const float*    vw_bd;              /* [num_atypes^2] B (vdW) or D (H-bond)   */
+    const float*    ds_s;               /* [num_atypes] solvation parameter       */
+    const float*    ds_v;               /* [num_atypes] atomic volume             */
+} adgpu_ligand;
+
+/* Scoring-function scalars (GpuDockparameters, cuda/GpuData.h:74-110;
+ * set at host/src/performdocking.cpp.Cuda:427-441). */
+typedef struct adgpu_energy_params {
+    float coeff_elec;          /* scaled_AD4_coeff_elec                      */
+    float coeff_desolv;        /* AD4_coeff_desolv                           */
+    float qasp;                /* 0.01097 by default                         */
+    float smooth;              /* 0.5 A by default                           */
+    float elec_min_distance;   /* 0.01 A by default                          */
+} adgpu_energy_params;
+
+/* Search parameters (Dockpars defaults at host/inc/getparameters.h:66-145;
+ * device scaling at host/src/performdocking.cpp.Cuda:427-463).  Rates are
+ * given in percent exactly as Dockpars holds them. */
+typedef struct adgpu_search_params {
+    int32_t  num_runs;
+    int32_t  pop_size;
+    int64_t  max_evals;            /* num_of_energy_evals, per run on average     */
+    int64_t  max_generations;      /* num_of_generations                           */
+    float    tournament_rate;      /* percent                                      */
+    float    crossover_rate;       /* percent                                      */
+    float    mutation_rate;        /* percent                                      */
+    float    abs_max_dmov;         /* grid units                                   */
+    float    abs_max_dang;         /* degrees                                      */
+    float    lsearch_rate;         /* percent                                      */
+    int32_t  ls_method;            /* adgpu_ls_method                              */
+    int32_t  initial_sw_generations;
+    float    rho_lower_bound;
+    float    base_dmov_mul_sqrt3;  /* grid units                                   */
+    float    base_dang_mul_sqrt3;  /* degrees                                      */
+    int32_t  cons_limit;
+    int32_t  max_num_of_iters;
+    float    adam_beta1, adam_beta2, adam_epsilon;
+} adgpu_search_params;
+
+typedef struct adgpu_context* adgpu_ctx;
+typedef struct adgpu_job*     adgpu_job_t;
+
+/* --- context ---------------------------------------------------------------
+ * Replaces setup_gpu_for_docking / finish_gpu_from_docking
+ * (host/inc/performdocking.h.Cuda:65-75, host/src/performdocking.cpp.Cuda:117-222). */
+int  adgpu_open(int device, adgpu_ctx* out);
+int  adgpu_close(adgpu_ctx ctx);
+const char* adgpu_last_error(adgpu_ctx ctx);
+/* Version string; also usable without a GPU. */
+const char* adgpu_version(void);
+
+/* --- resident inputs ---------------------------------------------------------
+ * Replace the per-job uploads in docking_with_gpu
+ * (host/src/performdocking.cpp.Cuda:363-410).  Maps and ligands stay resident
+ * until released, so several jobs can share one map set. */
+int  adgpu_maps_upload(adgpu_ctx ctx, const adgpu_grid* grid, int format, int* out_map_id);
+int  adgpu_maps_release(adgpu_ctx ctx, int map_id);
+int  adgpu_ligand_upload(adgpu_ctx ctx, const adgpu_ligand* lig,
+                         const adgpu_energy_params* ep, int* out_ligand_id);
+int  adgpu_ligand_release(adgpu_ctx ctx, int ligand_id);
+
+/* --- pose evaluation (the per-pose parity entry point) ------------------------
+ * Evaluates n host genotypes (stride ADGPU_GENOTYPE_STRIDE).  mode selects the
+ * reference device function: energy only (device/calcenergy.cl:132-554),
+ * merged energy + gradient (device/calcMergedEneGra.cl:46-936) or the float
+ * gradient used for FIRE's first step (device/calcgradient.cl:109-1003).
+ * energies[n] is written for ENERGY and ENERGRAD; grads[n*64] for ENERGRAD
+ * and GRADIENT (either may be NULL when not wanted). */
+int  adgpu_eval_poses(adgpu_ctx ctx, int map_id, int ligand_id, int mode,
+                      const float* genotypes, int n, float* energies, float* grads);
+
+/* --- docking jobs --------------------------------------------------------------
+ * A job owns the device state docking_with_gpu allocates per ligand
+ * (host/src/performdocking.cpp.Cuda:325-342): two population/energy buffers,
+ * per-entity eval counters, per-run eval totals and the LCG states.
+ * init_population: num_runs*pop_size*64 floats (gen_initpop_and_reflig,
+ * host/src/getparameters.cpp:2543-2622).  prng_seeds: num_runs*pop_size*64
+ * uint32 (host/src/performdocking.cpp.Cuda:310-317). */
+int  adgpu_job_create(adgpu_ctx ctx, int map_id, int ligand_id,
+                      const adgpu_search_params* sp,
+                      const float* init_population, const uint32_t* prng_seeds,
+                      adgpu_job_t* out);
+int  adgpu_job_destroy(adgpu_job_t job);
+
+/* K1 + K2: energies of the current population and the first eval counts
+ * (gpu_calc_initpop, device/kernel1.cl:28-145; gpu_sum_evals, device/kernel2.cl:26-62). */
+int  adgpu_job_initpop(adgpu_job_t job);
+/* K4 alone: elitism, selection, crossover, mutation and offspring energies from
+ * the current into the next buffers (gpu_gen_and_eval_newpops, device/kernel4.cl:32-336). */
+int  adgpu_job_kernel_gen(adgpu_job_t job);
+/* The local-search launch alone, on the next buffers, with the job's method
+ * (perform_LS kernel3.cl:30-436, gradient_minAD kernel_ad.cl:60-549,
+ * gpu_gradient_minAdam kernel_adam.cu:53-416, gradient_minFire kernel_fire.cl:75-895). */
+int  adgpu_job_kernel_ls(adgpu_job_t job, int generation);
+/* K2 alone, then the pointer flip (host/src/performdocking.cpp.Cuda:792-824). */
+int  adgpu_job_kernel_sum_and_swap(adgpu_job_t job);
+/* One whole generation: K4, LS, K2, flip (host/src/performdocking.cpp.Cuda:685-831). */
+int  adgpu_job_generation(adgpu_job_t job);
+/* Runs generations until check_progress reaches 100 % (host/src/performdocking.cpp.Cuda:
+ * 685, 908-942).  Returns the number of generations in *out_generations. */
+int  adgpu_job_run(adgpu_job_t job, int64_t* out_generations, int64_t* out_total_evals);
+/* Host copies of the current population [runs*pop*64], energies [runs*pop],
+ * per-run eval totals [runs] and LCG states [runs*pop*64]; any may be NULL.
+ * Mirrors the final downloads at host/src/performdocking.cpp.Cuda:856-859. */
+int  adgpu_job_download(adgpu_job_t job, float* population, float* energies,
+                        int32_t* evals_of_runs, uint32_t* prng_states);
+/* Overwrites the current population, energies and LCG states (any may be NULL). */
+int  adgpu_job_upload(adgpu_job_t job, const float* population, const float* energies,
+                      const uint32_t* prng_states);
+/* Average device time in milliseconds of the last launches of one kernel
+ * class, measured with HIP events on the job's stream (0 = gen, 1 = ls). */
+int  adgpu_job_kernel_time(adgpu_job_t job, int kernel_class, double* avg_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADGPU_H_ */

@@ -1,0 +1,109 @@
+/*
+ * adock_oracle.h -- CPU restatement of AutoDock-GPU's docking hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP path in
+ * autodock-gpu_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  Nothing in the product links or calls it.
+ *
+ * adock_oracle.c is compiled twice from one source:
+ *   *_f32  -- FP32 restatement of the reference device arithmetic, including
+ *             the int32 fixed-point gradient quantization and its rounding;
+ *   *_f64  -- the same formulas in FP64; quantization is a run-time switch.
+ * The GA / local-search / LGA functions exist only in the FP32 build: they
+ * mirror the device control flow and the per-work-item LCG streams.
+ *
+ * Parity pinning of each function (which tests fix it) is listed in the
+ * header comment of adock_oracle.c.
+ */
+#ifndef ADOCK_ORACLE_H_
+#define ADOCK_ORACLE_H_
+
+#include "../include/adgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Energy of one genotype.  calcenergy.cl:132-554.  inter/intra/abs_sum may
+ * be NULL; abs_sum receives the sum of |term| (the scale FP32 rounding error
+ * is proportional to). */
+float  orc_energy_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                      const float* genotype, float* inter, float* intra, float* abs_sum);
+double orc_energy_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                      const float* genotype, double* inter, double* intra, double* abs_sum);
+
+/* Merged energy + gradient.  calcMergedEneGra.cl:46-936.  grad receives
+ * num_rotbonds+6 values.  quantize=1 applies the int32 fixed-point rounding
+ * (TERMSCALE 1024, per-term clamp +-8192, calcMergedEneGra.cl:25-35). */
+float  orc_energrad_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                        const float* genotype, int quantize, float* grad);
+double orc_energrad_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                        const float* genotype, int quantize, double* grad);
+
+/* Float gradient used by FIRE's first step.  calcgradient.cl:109-1003. */
+void   orc_gradient_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                        const float* genotype, float* grad);
+void   orc_gradient_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                        const float* genotype, double* grad);
+
+/* Atom coordinates (grid units) after pose->coordinates.  calcenergy.cl:185-275. */
+void   orc_coords_f32(const adgpu_ligand* L, const float* genotype, float* xyz);
+void   orc_coords_f64(const adgpu_ligand* L, const float* genotype, double* xyz);
+
+/* ---- FP32-only: device control flow ------------------------------------ */
+
+/* One LCG draw from state *s (auxiliary_genetic.cl:29-67). */
+uint32_t orc_rand(uint32_t* s);
+float    orc_randf(uint32_t* s);
+
+/* Job state, laid out as the reference's device buffers. */
+typedef struct orc_job {
+    const adgpu_grid*          grid;
+    const adgpu_ligand*        lig;
+    const adgpu_energy_params* ep;
+    adgpu_search_params        sp;
+    float*    pop_cur;        /* runs*pop*64 */
+    float*    pop_next;
+    float*    ene_cur;        /* runs*pop    */
+    float*    ene_next;
+    int32_t*  evals_new;      /* runs*pop    */
+    int32_t*  evals_runs;     /* runs        */
+    uint32_t* prng;           /* runs*pop*64 */
+    int64_t   generation;
+} orc_job;
+
+/* Allocates the buffers and copies the initial population and seeds. */
+int  orc_job_init(orc_job* j, const adgpu_grid* g, const adgpu_ligand* L,
+                  const adgpu_energy_params* ep, const adgpu_search_params* sp,
+                  const float* init_pop, const uint32_t* seeds);
+void orc_job_free(orc_job* j);
+/* K1 + K2 for runs [r0, r1). */
+void orc_initpop(orc_job* j, int r0, int r1);
+/* K4 for runs [r0, r1): current -> next.  kernel4.cl:32-336. */
+void orc_gen(orc_job* j, int r0, int r1);
+/* Local search on next for runs [r0, r1), generation index gen. */
+void orc_ls(orc_job* j, int r0, int r1, int64_t gen);
+/* K2 for runs [r0, r1). kernel2.cl:26-62. */
+void orc_sum_evals(orc_job* j, int r0, int r1);
+/* Swaps current and next. */
+void orc_swap(orc_job* j);
+/* Whole LGA until check_progress >= 100 (performdocking.cpp.Cuda:685-831,
+ * 908-942).  Runs are independent; nthreads > 1 splits them over OpenMP
+ * threads (each run is still computed serially).  Returns generations. */
+int64_t orc_lga_run(orc_job* j, int nthreads, int64_t* total_evals);
+/* Runs a fixed number of generations on runs [r0, r1) only (for bounded
+ * CPU-baseline samples); returns the evals performed. */
+int64_t orc_lga_generations(orc_job* j, int r0, int r1, int64_t ngen, int nthreads);
+
+/* Host CMWC4096 generator with the reference's seeding (miscellaneous.h:146-218):
+ * fills n uint32 values from seed[3]. */
+void orc_cmwc_uints(const uint32_t seed[3], uint32_t* out, int64_t n);
+/* Initial population as gen_initpop_and_reflig draws it
+ * (getparameters.cpp:2587-2622): runs*pop genotypes of 64 floats. */
+void orc_initial_population(const uint32_t seed[3], int runs, int pop,
+                            const float size_xyz_angstr[3], float spacing, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
