@@ -1,0 +1,176 @@
+"""adgpu -- MI355X (gfx950) docking hot path of AutoDock-GPU behind a C ABI.
+
+Python mirror of the host interface (include/adgpu.h).  The compute path is
+the HIP library libadgpu_hip.so; this module only marshals arrays.  There is
+no CPU fallback: every call raises if the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (EVAL_ENERGRAD, EVAL_ENERGY, EVAL_GRADIENT, GENOTYPE_STRIDE, LS_AD, LS_ADAM, LS_FIRE, LS_NONE,
+                   LS_SW, MAP_BF16, MAP_F16, MAP_F32, NWI, CGrid, CLigand, energy_params, search_params)
+
+__all__ = ["Context", "Job", "AdgpuError", "energy_params", "search_params", "MAP_F32", "MAP_F16", "MAP_BF16",
+           "EVAL_ENERGY", "EVAL_ENERGRAD", "EVAL_GRADIENT", "LS_NONE", "LS_SW", "LS_AD", "LS_ADAM", "LS_FIRE",
+           "GENOTYPE_STRIDE", "NWI"]
+
+
+class AdgpuError(RuntimeError):
+    pass
+
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Context:
+    """A device, its stream, and the resident map sets and ligands
+    (replaces setup_gpu_for_docking / finish_gpu_from_docking,
+    host/src/performdocking.cpp.Cuda:117-222)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load_library()
+        h = C.c_void_p()
+        rc = self.lib.adgpu_open(device, C.byref(h))
+        if rc != 0:
+            raise AdgpuError("adgpu_open(%d) failed with status %d (no usable GPU?)" % (device, rc))
+        self.h = h
+        self._keep = {}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.adgpu_last_error(self.h)
+            raise AdgpuError("%s failed (status %d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adgpu_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def maps_upload(self, packed, size, num_map_atypes, spacing, fmt=MAP_F32) -> int:
+        g = CGrid(packed, size, num_map_atypes, spacing)
+        mid = C.c_int()
+        self._check(self.lib.adgpu_maps_upload(self.h, C.byref(g.s), int(fmt), C.byref(mid)), "adgpu_maps_upload")
+        return mid.value
+
+    def maps_release(self, map_id: int):
+        self._check(self.lib.adgpu_maps_release(self.h, map_id), "adgpu_maps_release")
+
+    def ligand_upload(self, tables, ep=None) -> int:
+        cl = CLigand(tables)
+        ep = ep or energy_params()
+        lid = C.c_int()
+        self._check(self.lib.adgpu_ligand_upload(self.h, C.byref(cl.s), C.byref(ep), C.byref(lid)),
+                    "adgpu_ligand_upload")
+        return lid.value
+
+    def ligand_release(self, ligand_id: int):
+        self._check(self.lib.adgpu_ligand_release(self.h, ligand_id), "adgpu_ligand_release")
+
+    def eval_poses(self, map_id, ligand_id, genotypes, mode=EVAL_ENERGY):
+        g = _f32(genotypes)
+        if g.ndim != 2 or g.shape[1] != GENOTYPE_STRIDE:
+            raise ValueError("genotypes must be (n, 64)")
+        n = g.shape[0]
+        e = np.zeros(n, np.float32)
+        gr = np.zeros((n, GENOTYPE_STRIDE), np.float32)
+        self._check(self.lib.adgpu_eval_poses(self.h, map_id, ligand_id, int(mode), g.ctypes.data_as(_fp), n,
+                                              e.ctypes.data_as(_fp), gr.ctypes.data_as(_fp)), "adgpu_eval_poses")
+        return e, gr
+
+    def job(self, map_id, ligand_id, sp, init_pop, seeds) -> "Job":
+        return Job(self, map_id, ligand_id, sp, init_pop, seeds)
+
+
+class Job:
+    """Device state of one docking (docking_with_gpu's per-ligand buffers)."""
+
+    def __init__(self, ctx: Context, map_id, ligand_id, sp, init_pop, seeds):
+        self.ctx, self.lib, self.sp = ctx, ctx.lib, sp
+        self.ne = sp.num_runs * sp.pop_size
+        ip = _f32(init_pop).reshape(-1)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1)
+        if ip.size != self.ne * GENOTYPE_STRIDE or sd.size != self.ne * NWI:
+            raise ValueError("init_pop / seeds sizes do not match num_runs * pop_size")
+        h = C.c_void_p()
+        ctx._check(self.lib.adgpu_job_create(ctx.h, map_id, ligand_id, C.byref(sp), ip.ctypes.data_as(_fp),
+                                             sd.ctypes.data_as(_up), C.byref(h)), "adgpu_job_create")
+        self.h = h
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            self.lib.adgpu_job_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        self.ctx._check(rc, what)
+
+    def initpop(self):
+        self._c(self.lib.adgpu_job_initpop(self.h), "adgpu_job_initpop")
+
+    def kernel_gen(self):
+        self._c(self.lib.adgpu_job_kernel_gen(self.h), "adgpu_job_kernel_gen")
+
+    def kernel_ls(self, generation=0):
+        self._c(self.lib.adgpu_job_kernel_ls(self.h, generation), "adgpu_job_kernel_ls")
+
+    def sum_and_swap(self):
+        self._c(self.lib.adgpu_job_kernel_sum_and_swap(self.h), "adgpu_job_kernel_sum_and_swap")
+
+    def generation(self):
+        self._c(self.lib.adgpu_job_generation(self.h), "adgpu_job_generation")
+
+    def run(self):
+        g, t = C.c_int64(), C.c_int64()
+        self._c(self.lib.adgpu_job_run(self.h, C.byref(g), C.byref(t)), "adgpu_job_run")
+        return g.value, t.value
+
+    def download(self):
+        pop = np.zeros((self.ne, GENOTYPE_STRIDE), np.float32)
+        ene = np.zeros(self.ne, np.float32)
+        ev = np.zeros(self.sp.num_runs, np.int32)
+        pr = np.zeros(self.ne * NWI, np.uint32)
+        self._c(self.lib.adgpu_job_download(self.h, pop.ctypes.data_as(_fp), ene.ctypes.data_as(_fp),
+                                            ev.ctypes.data_as(_ip), pr.ctypes.data_as(_up)), "adgpu_job_download")
+        return dict(pop=pop, ene=ene, evals_runs=ev, prng=pr)
+
+    def upload(self, pop=None, ene=None, prng=None):
+        p = _f32(pop).reshape(-1) if pop is not None else None
+        e = _f32(ene).reshape(-1) if ene is not None else None
+        r = np.ascontiguousarray(prng, dtype=np.uint32).reshape(-1) if prng is not None else None
+        self._c(self.lib.adgpu_job_upload(self.h, p.ctypes.data_as(_fp) if p is not None else None,
+                                          e.ctypes.data_as(_fp) if e is not None else None,
+                                          r.ctypes.data_as(_up) if r is not None else None), "adgpu_job_upload")
+
+    def kernel_time(self, cls):
+        ms, n = C.c_double(), C.c_int64()
+        self._c(self.lib.adgpu_job_kernel_time(self.h, cls, C.byref(ms), C.byref(n)), "adgpu_job_kernel_time")
+        return ms.value, n.value

@@ -1,0 +1,616 @@
+// adgpu_host.cpp -- host side of the C ABI declared in include/adgpu.h.
+//
+// Owns the HIP context objects (one stream per context, default priority),
+// the resident map sets and ligands, and the docking jobs.  The generation
+// loop of adgpu_job_run follows docking_with_gpu
+// (host/src/performdocking.cpp.Cuda:685-831): K4, the local search, K2 and the
+// population pointer flip, stopped by check_progress (:908-942).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "adgpu_internal.h"
+
+using namespace adgpu;
+
+namespace {
+
+struct MapSet {
+    GridDev dev{};
+    void* data = nullptr;
+    size_t bytes = 0;
+};
+
+struct LigandSet {
+    LigandDev host{};
+    void* dev = nullptr;
+    int max_map = 0;      // largest atom_map index (must be < the grid's type maps)
+};
+
+}  // namespace
+
+struct adgpu_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<int, MapSet> maps;
+    std::map<int, LigandSet> ligands;
+    int next_id = 1;
+    std::string err;
+    std::mutex mu;
+    // scratch for adgpu_eval_poses
+    float* d_geno = nullptr;
+    float* d_ene = nullptr;
+    float* d_grad = nullptr;
+    int scratch_n = 0;
+};
+
+struct adgpu_job {
+    adgpu_context* ctx = nullptr;
+    int map_id = 0, ligand_id = 0;
+    SearchDev sd{};
+    adgpu_search_params sp{};
+    JobBuffers b{};
+    int32_t* h_evals_runs = nullptr;   // pinned
+    int64_t generation = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double kernel_ms[2] = {0.0, 0.0};
+    int64_t kernel_n[2] = {0, 0};
+};
+
+namespace {
+
+int fail(adgpu_context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail((ctx), ADGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+uint16_t f32_to_f16(float f) {
+    _Float16 h = (_Float16)f;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+uint16_t f32_to_bf16(float f) {   // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+// Builds the single device block of one ligand (see LigandDev).
+int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                 std::vector<char>& blob, LigandDev& hd) {
+    const int N = L->num_atoms, T = L->num_rotbonds, NT = L->num_atypes, P = L->num_pairs;
+    if (N <= 0 || N > ADGPU_MAX_ATOMS) return fail(c, ADGPU_ERR_LIMIT, "num_atoms out of range (1..256)");
+    if (T < 0 || T > ADGPU_MAX_ROTBONDS) return fail(c, ADGPU_ERR_LIMIT, "num_rotbonds out of range (0..57)");
+    if (NT <= 0 || NT > ADGPU_MAX_ATYPES) return fail(c, ADGPU_ERR_LIMIT, "num_atypes out of range (1..32)");
+    if (P < 0 || P > N * (N - 1) / 2) return fail(c, ADGPU_ERR_LIMIT, "num_pairs out of range");
+    if (L->true_ligand_atoms < 0 || L->true_ligand_atoms > N) return fail(c, ADGPU_ERR_ARG, "true_ligand_atoms out of range");
+    if (!L->charges || !L->atom_types || !L->atom_map || !L->ref_coords || (P && !L->pairs) ||
+        !L->atype_reqm || !L->vw_exp || !L->reqm_ab || !L->vw_ac || !L->vw_bd || !L->ds_s || !L->ds_v ||
+        (T && (!L->rotbond_moving || !L->rotbond_unit || !L->rotbond_atoms || !L->atom_rotbonds)))
+        return fail(c, ADGPU_ERR_ARG, "null ligand array");
+    std::memset(&hd, 0, sizeof(hd));
+    hd.num_atoms = N;
+    hd.true_ligand_atoms = L->true_ligand_atoms;
+    hd.num_rotbonds = T;
+    hd.num_genes = T + 6;
+    hd.num_pairs = P;
+    hd.nchunk = N <= 64 ? 1 : (N <= 128 ? 2 : 4);
+    hd.coeff_elec = ep->coeff_elec;
+    hd.coeff_desolv = ep->coeff_desolv;
+    hd.qasp = ep->qasp;
+    hd.half_smooth = 0.5f * ep->smooth;
+    hd.elec_min_distance = ep->elec_min_distance;
+    size_t off = (sizeof(LigandDev) + 255) & ~size_t(255);
+    auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) & ~size_t(255); return (uint32_t)o; };
+    hd.off_atom_ref = take(sizeof(float4) * N);
+    hd.off_atom_info = take(sizeof(int4) * N);
+    hd.off_atom_mask = take(sizeof(uint2) * N);
+    hd.off_tor_unit = take(sizeof(float4) * (T ? T : 1));
+    hd.off_tor_move = take(sizeof(float4) * (T ? T : 1));
+    hd.off_tor_atoms = take(sizeof(int2) * (T ? T : 1));
+    hd.off_pairs = take(sizeof(PairRec) * (P ? P : 1));
+    hd.total_bytes = (uint32_t)off;
+    blob.assign(off, 0);
+    float4* ar = reinterpret_cast<float4*>(blob.data() + hd.off_atom_ref);
+    int4* ai = reinterpret_cast<int4*>(blob.data() + hd.off_atom_info);
+    uint2* am = reinterpret_cast<uint2*>(blob.data() + hd.off_atom_mask);
+    for (int a = 0; a < N; ++a) {
+        if (L->atom_types[a] < 0 || L->atom_types[a] >= NT) return fail(c, ADGPU_ERR_ARG, "atom type out of range");
+        ar[a] = make_float4(L->ref_coords[3 * a], L->ref_coords[3 * a + 1], L->ref_coords[3 * a + 2], L->charges[a]);
+        ai[a] = make_int4(L->atom_map[a], L->atom_types[a], L->ignore_inter ? (int)L->ignore_inter[a] : 0, 0);
+        uint64_t m = 0;
+        for (int t = 0; t < T; ++t)
+            if (L->atom_rotbonds[(size_t)a * T + t]) m |= (uint64_t)1 << t;
+        am[a] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+    }
+    float4* tu = reinterpret_cast<float4*>(blob.data() + hd.off_tor_unit);
+    float4* tm = reinterpret_cast<float4*>(blob.data() + hd.off_tor_move);
+    int2* ta = reinterpret_cast<int2*>(blob.data() + hd.off_tor_atoms);
+    for (int t = 0; t < T; ++t) {
+        tu[t] = make_float4(L->rotbond_unit[3 * t], L->rotbond_unit[3 * t + 1], L->rotbond_unit[3 * t + 2], 0.f);
+        tm[t] = make_float4(L->rotbond_moving[3 * t], L->rotbond_moving[3 * t + 1], L->rotbond_moving[3 * t + 2], 0.f);
+        int a0 = L->rotbond_atoms[2 * t], a1 = L->rotbond_atoms[2 * t + 1];
+        if (a0 < 0 || a0 >= N || a1 < 0 || a1 >= N) return fail(c, ADGPU_ERR_ARG, "rotbond atom out of range");
+        ta[t] = make_int2(a0, a1);
+    }
+    PairRec* pr = reinterpret_cast<PairRec*>(blob.data() + hd.off_pairs);
+    for (int p = 0; p < P; ++p) {
+        int i = L->pairs[2 * p], j = L->pairs[2 * p + 1];
+        if (i < 0 || i >= N || j < 0 || j >= N) return fail(c, ADGPU_ERR_ARG, "pair atom out of range");
+        int t1 = L->atom_types[i], t2 = L->atom_types[j];
+        int idx = t1 * NT + t2;
+        uint32_t m = (L->vw_exp[idx] >> 8) & 0xffu, n = L->vw_exp[idx] & 0xffu;
+        uint32_t r1 = L->atype_reqm[t1], r2 = L->atype_reqm[t2];
+        bool cgg0 = (r1 == 24 && r2 == 25) || (r1 == 25 && r2 == 24);   // ATYPE_CG_IDX / ATYPE_G0_IDX
+        bool fast = (m == 12) && (n == 6 || n == 10);
+        pr[p].ij = (uint32_t)i | ((uint32_t)j << 16);
+        pr[p].flags = m | (n << 8) | ((uint32_t)cgg0 << 16) | ((uint32_t)fast << 17);
+        pr[p].ac = L->vw_ac[idx];
+        pr[p].bd = L->vw_bd[idx];
+        pr[p].reqm = L->reqm_ab[idx];
+        float q1 = L->charges[i], q2 = L->charges[j];
+        pr[p].cq = ep->coeff_elec * q1 * q2;                       // calcenergy.cl:527 order
+        pr[p].dsol = (L->ds_s[t1] + ep->qasp * std::fabs(q1)) * L->ds_v[t2] +
+                     (L->ds_s[t2] + ep->qasp * std::fabs(q2)) * L->ds_v[t1];   // calcenergy.cl:488-491
+        pr[p].pad = 0.f;
+    }
+    std::memcpy(blob.data(), &hd, sizeof(hd));
+    return ADGPU_OK;
+}
+
+int check_search(adgpu_context* c, const adgpu_search_params* sp, int num_genes) {
+    if (sp->num_runs <= 0 || sp->num_runs > ADGPU_MAX_RUNS) return fail(c, ADGPU_ERR_LIMIT, "num_runs out of range");
+    if (sp->pop_size <= 1 || sp->pop_size > ADGPU_MAX_POPSIZE) return fail(c, ADGPU_ERR_LIMIT, "pop_size out of range");
+    if (sp->ls_method < ADGPU_LS_NONE || sp->ls_method > ADGPU_LS_FIRE) return fail(c, ADGPU_ERR_ARG, "bad ls_method");
+    if (sp->max_num_of_iters < 0) return fail(c, ADGPU_ERR_ARG, "max_num_of_iters < 0");
+    (void)num_genes;
+    return ADGPU_OK;
+}
+
+SearchDev to_dev(const adgpu_search_params* sp, int num_genes) {
+    SearchDev s{};
+    s.num_runs = sp->num_runs;
+    s.pop_size = sp->pop_size;
+    s.num_genes = num_genes;
+    s.num_ls = sp->lsearch_rate != 0.0f ? (int)(unsigned)(sp->lsearch_rate / 100.0 * sp->pop_size + 0.5) : 0;
+    s.tournament_rate = sp->tournament_rate / 100.0f;
+    s.crossover_rate = sp->crossover_rate / 100.0f;
+    s.mutation_rate = sp->mutation_rate / 100.f;
+    s.abs_max_dmov = sp->abs_max_dmov;
+    s.abs_max_dang = sp->abs_max_dang;
+    s.lsearch_rate = sp->lsearch_rate;
+    s.rho_lower_bound = sp->rho_lower_bound;
+    s.base_dmov_mul_sqrt3 = sp->base_dmov_mul_sqrt3;
+    s.base_dang_mul_sqrt3 = sp->base_dang_mul_sqrt3;
+    s.cons_limit = sp->cons_limit;
+    s.max_iters = sp->max_num_of_iters;
+    s.ls_method = sp->ls_method;
+    s.adam_beta1 = sp->adam_beta1;
+    s.adam_beta2 = sp->adam_beta2;
+    s.adam_epsilon = sp->adam_epsilon;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* adgpu_version(void) { return "adgpu-mi355x 0.1 (gfx950)"; }
+
+const char* adgpu_last_error(adgpu_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int adgpu_open(int device, adgpu_ctx* out) {
+    if (!out) return ADGPU_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return ADGPU_ERR_HIP;
+    if (device < 0) device = 0;
+    if (device >= n) return ADGPU_ERR_ARG;
+    std::unique_ptr<adgpu_context> c(new adgpu_context());
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return ADGPU_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
+    *out = c.release();
+    return ADGPU_OK;
+}
+
+int adgpu_close(adgpu_ctx c) {
+    if (!c) return ADGPU_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->maps) (void)hipFree(kv.second.data);
+    for (auto& kv : c->ligands) (void)hipFree(kv.second.dev);
+    (void)hipFree(c->d_geno);
+    (void)hipFree(c->d_ene);
+    (void)hipFree(c->d_grad);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ADGPU_OK;
+}
+
+int adgpu_maps_upload(adgpu_ctx c, const adgpu_grid* g, int format, int* out_id) {
+    if (!c || !g || !out_id || !g->packed) return fail(c, ADGPU_ERR_ARG, "null argument");
+    if (g->size_x < 2 || g->size_y < 2 || g->size_z < 2 || g->size_x > ADGPU_MAX_GRIDPOINTS ||
+        g->size_y > ADGPU_MAX_GRIDPOINTS || g->size_z > ADGPU_MAX_GRIDPOINTS)
+        return fail(c, ADGPU_ERR_LIMIT, "grid size out of range (2..256 points per axis)");
+    if (g->num_map_atypes <= 0 || g->num_map_atypes > ADGPU_MAX_ATYPES)
+        return fail(c, ADGPU_ERR_LIMIT, "num_map_atypes out of range");
+    if (!(g->spacing > 0.f) || g->spacing > 1.0f) return fail(c, ADGPU_ERR_ARG, "spacing must be in (0, 1]");
+    if (format != ADGPU_MAP_F32 && format != ADGPU_MAP_F16 && format != ADGPU_MAP_BF16)
+        return fail(c, ADGPU_ERR_ARG, "unknown map format");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    MapSet ms;
+    const size_t g3 = (size_t)g->size_x * g->size_y * g->size_z;
+    const size_t nmaps = (size_t)g->num_map_atypes + 2;
+    const size_t nval = 4 * g3 * nmaps;
+    std::vector<uint16_t> half;
+    const void* src = g->packed;
+    size_t bytes = nval * sizeof(float);
+    if (format != ADGPU_MAP_F32) {
+        half.resize(nval);
+        for (size_t i = 0; i < nval; ++i)
+            half[i] = format == ADGPU_MAP_F16 ? f32_to_f16(g->packed[i]) : f32_to_bf16(g->packed[i]);
+        src = half.data();
+        bytes = nval * sizeof(uint16_t);
+    }
+    if (hipMalloc(&ms.data, bytes) != hipSuccess) return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(maps) failed");
+    hipError_t e = hipMemcpy(ms.data, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(ms.data); return fail(c, ADGPU_ERR_HIP, hipGetErrorString(e)); }
+    ms.bytes = bytes;
+    GridDev& d = ms.dev;
+    d.gx = g->size_x; d.gy = g->size_y; d.gz = g->size_z;
+    d.nmaps = (int)nmaps;
+    d.g1 = (uint32_t)g->size_x;
+    d.g2 = (uint32_t)g->size_x * g->size_y;
+    d.g3 = (uint32_t)g3;
+    d.format = (uint32_t)format;
+    d.spacing = g->spacing;
+    d.inv_spacing = 1.0f / g->spacing;
+    d.data = ms.data;
+    int id = c->next_id++;
+    c->maps[id] = ms;
+    *out_id = id;
+    return ADGPU_OK;
+}
+
+int adgpu_maps_release(adgpu_ctx c, int id) {
+    if (!c) return ADGPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->maps.find(id);
+    if (it == c->maps.end()) return fail(c, ADGPU_ERR_ARG, "unknown map id");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(it->second.data);
+    c->maps.erase(it);
+    return ADGPU_OK;
+}
+
+int adgpu_ligand_upload(adgpu_ctx c, const adgpu_ligand* L, const adgpu_energy_params* ep, int* out_id) {
+    if (!c || !L || !ep || !out_id) return fail(c, ADGPU_ERR_ARG, "null argument");
+    std::vector<char> blob;
+    LigandDev hd;
+    int rc = build_ligand(c, L, ep, blob, hd);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    LigandSet ls;
+    ls.host = hd;
+    for (int a = 0; a < L->num_atoms; ++a)
+        if (!(L->ignore_inter && L->ignore_inter[a]) && L->atom_map[a] > ls.max_map) ls.max_map = L->atom_map[a];
+    for (int a = 0; a < L->num_atoms; ++a)
+        if (L->atom_map[a] < 0 && !(L->ignore_inter && L->ignore_inter[a]))
+            return fail(c, ADGPU_ERR_ARG, "negative atom_map index");
+    if (hipMalloc(&ls.dev, blob.size()) != hipSuccess) return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(ligand) failed");
+    hipError_t e = hipMemcpy(ls.dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(ls.dev); return fail(c, ADGPU_ERR_HIP, hipGetErrorString(e)); }
+    int id = c->next_id++;
+    c->ligands[id] = ls;
+    *out_id = id;
+    return ADGPU_OK;
+}
+
+int adgpu_ligand_release(adgpu_ctx c, int id) {
+    if (!c) return ADGPU_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->ligands.find(id);
+    if (it == c->ligands.end()) return fail(c, ADGPU_ERR_ARG, "unknown ligand id");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(it->second.dev);
+    c->ligands.erase(it);
+    return ADGPU_OK;
+}
+
+int adgpu_eval_poses(adgpu_ctx c, int map_id, int lig_id, int mode, const float* genotypes, int n,
+                     float* energies, float* grads) {
+    if (!c || !genotypes || n < 0) return fail(c, ADGPU_ERR_ARG, "null argument");
+    if (mode < ADGPU_EVAL_ENERGY || mode > ADGPU_EVAL_GRADIENT) return fail(c, ADGPU_ERR_ARG, "bad mode");
+    if (n == 0) return ADGPU_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto mi = c->maps.find(map_id);
+    auto li = c->ligands.find(lig_id);
+    if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_ARG, "unknown map or ligand id");
+    if (li->second.max_map >= mi->second.dev.nmaps - 2)
+        return fail(c, ADGPU_ERR_ARG, "ligand atom_map index exceeds the grid's type maps");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n > c->scratch_n) {
+        (void)hipFree(c->d_geno); (void)hipFree(c->d_ene); (void)hipFree(c->d_grad);
+        c->d_geno = c->d_ene = c->d_grad = nullptr;
+        c->scratch_n = 0;
+        if (hipMalloc(&c->d_geno, sizeof(float) * kGenes * (size_t)n) != hipSuccess ||
+            hipMalloc(&c->d_ene, sizeof(float) * (size_t)n) != hipSuccess ||
+            hipMalloc(&c->d_grad, sizeof(float) * kGenes * (size_t)n) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(eval scratch) failed");
+        c->scratch_n = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_geno, genotypes, sizeof(float) * kGenes * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_eval_poses(c->stream, (const LigandDev*)li->second.dev, li->second.host, mi->second.dev, mode,
+                                c->d_geno, n, c->d_ene, c->d_grad));
+    if (energies && mode != ADGPU_EVAL_GRADIENT)
+        HIPCHK(c, hipMemcpyAsync(energies, c->d_ene, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    if (grads && mode != ADGPU_EVAL_ENERGY)
+        HIPCHK(c, hipMemcpyAsync(grads, c->d_grad, sizeof(float) * kGenes * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADGPU_OK;
+}
+
+int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_params* sp, const float* init_pop,
+                     const uint32_t* seeds, adgpu_job_t* out) {
+    if (!c || !sp || !init_pop || !seeds || !out) return fail(c, ADGPU_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto mi = c->maps.find(map_id);
+    auto li = c->ligands.find(lig_id);
+    if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_ARG, "unknown map or ligand id");
+    if (li->second.max_map >= mi->second.dev.nmaps - 2)
+        return fail(c, ADGPU_ERR_ARG, "ligand atom_map index exceeds the grid's type maps");
+    int rc = check_search(c, sp, li->second.host.num_genes);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::unique_ptr<adgpu_job> j(new adgpu_job());
+    j->ctx = c;
+    j->map_id = map_id;
+    j->ligand_id = lig_id;
+    j->sp = *sp;
+    j->sd = to_dev(sp, li->second.host.num_genes);
+    if (j->sd.num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
+    const size_t ne = (size_t)sp->num_runs * sp->pop_size;
+    bool ok = hipMalloc(&j->b.pop_cur, ne * kGenes * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->b.pop_next, ne * kGenes * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->b.ene_cur, ne * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->b.ene_next, ne * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->b.evals_new, ne * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&j->b.evals_runs, sp->num_runs * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&j->b.prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess &&
+              hipHostMalloc(&j->h_evals_runs, sp->num_runs * sizeof(int32_t)) == hipSuccess;
+    if (!ok) {
+        adgpu_job_destroy(j.release());
+        return fail(c, ADGPU_ERR_NOMEM, "job allocation failed");
+    }
+    for (auto& e : j->ev)
+        if (hipEventCreate(&e) != hipSuccess) { adgpu_job_destroy(j.release()); return fail(c, ADGPU_ERR_HIP, "hipEventCreate"); }
+    HIPCHK(c, hipMemcpy(j->b.pop_cur, init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(j->b.pop_next, init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(j->b.prng, seeds, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(j->b.evals_runs, 0, sp->num_runs * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(j->b.evals_new, 0, ne * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(j->b.ene_cur, 0, ne * sizeof(float)));
+    HIPCHK(c, hipMemset(j->b.ene_next, 0, ne * sizeof(float)));
+    *out = j.release();
+    return ADGPU_OK;
+}
+
+int adgpu_job_destroy(adgpu_job_t j) {
+    if (!j) return ADGPU_ERR_ARG;
+    if (j->ctx) { (void)hipSetDevice(j->ctx->device); (void)hipStreamSynchronize(j->ctx->stream); }
+    (void)hipFree(j->b.pop_cur); (void)hipFree(j->b.pop_next);
+    (void)hipFree(j->b.ene_cur); (void)hipFree(j->b.ene_next);
+    (void)hipFree(j->b.evals_new); (void)hipFree(j->b.evals_runs); (void)hipFree(j->b.prng);
+    if (j->h_evals_runs) (void)hipHostFree(j->h_evals_runs);
+    for (auto& e : j->ev) if (e) (void)hipEventDestroy(e);
+    delete j;
+    return ADGPU_OK;
+}
+
+static int job_lookup(adgpu_job_t j, const LigandSet** ls, const MapSet** ms) {
+    adgpu_context* c = j->ctx;
+    auto mi = c->maps.find(j->map_id);
+    auto li = c->ligands.find(j->ligand_id);
+    if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_STATE, "map or ligand released");
+    *ls = &li->second;
+    *ms = &mi->second;
+    return ADGPU_OK;
+}
+
+int adgpu_job_initpop(adgpu_job_t j) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    const LigandSet* ls; const MapSet* ms;
+    int rc = job_lookup(j, &ls, &ms);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_initpop(c->stream, (const LigandDev*)ls->dev, ls->host, ms->dev, j->sd, j->b));
+    HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADGPU_OK;
+}
+
+static int run_gen(adgpu_job_t j, const LigandSet* ls, const MapSet* ms) {
+    adgpu_context* c = j->ctx;
+    HIPCHK(c, hipEventRecord(j->ev[0], c->stream));
+    HIPCHK(c, launch_gen(c->stream, (const LigandDev*)ls->dev, ls->host, ms->dev, j->sd, j->b));
+    HIPCHK(c, hipEventRecord(j->ev[1], c->stream));
+    return ADGPU_OK;
+}
+
+static int ls_method_for(adgpu_job_t j, int64_t generation) {
+    // performdocking.cpp.Cuda:727-731: AD falls back to SW for the first
+    // initial_sw_generations generations
+    int m = j->sp.ls_method;
+    if (j->sd.num_ls == 0 || j->sp.lsearch_rate == 0.0f) return ADGPU_LS_NONE;
+    if (m == ADGPU_LS_AD && generation < j->sp.initial_sw_generations) m = ADGPU_LS_SW;
+    return m;
+}
+
+static int run_ls(adgpu_job_t j, const LigandSet* ls, const MapSet* ms, int64_t generation) {
+    adgpu_context* c = j->ctx;
+    int m = ls_method_for(j, generation);
+    if (m == ADGPU_LS_NONE) return ADGPU_OK;
+    HIPCHK(c, hipEventRecord(j->ev[2], c->stream));
+    HIPCHK(c, launch_ls(c->stream, (const LigandDev*)ls->dev, ls->host, ms->dev, j->sd, j->b, m));
+    HIPCHK(c, hipEventRecord(j->ev[3], c->stream));
+    return ADGPU_OK;
+}
+
+static void swap_bufs(adgpu_job_t j) {
+    std::swap(j->b.pop_cur, j->b.pop_next);
+    std::swap(j->b.ene_cur, j->b.ene_next);
+}
+
+static void accumulate_times(adgpu_job_t j, bool had_ls) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, j->ev[0], j->ev[1]) == hipSuccess) { j->kernel_ms[0] += ms; j->kernel_n[0]++; }
+    if (had_ls && hipEventElapsedTime(&ms, j->ev[2], j->ev[3]) == hipSuccess) { j->kernel_ms[1] += ms; j->kernel_n[1]++; }
+}
+
+int adgpu_job_kernel_gen(adgpu_job_t j) {
+    if (!j) return ADGPU_ERR_ARG;
+    const LigandSet* ls; const MapSet* ms;
+    int rc = job_lookup(j, &ls, &ms);
+    if (rc) return rc;
+    HIPCHK(j->ctx, hipSetDevice(j->ctx->device));
+    rc = run_gen(j, ls, ms);
+    if (rc) return rc;
+    HIPCHK(j->ctx, hipStreamSynchronize(j->ctx->stream));
+    return ADGPU_OK;
+}
+
+int adgpu_job_kernel_ls(adgpu_job_t j, int generation) {
+    if (!j) return ADGPU_ERR_ARG;
+    const LigandSet* ls; const MapSet* ms;
+    int rc = job_lookup(j, &ls, &ms);
+    if (rc) return rc;
+    HIPCHK(j->ctx, hipSetDevice(j->ctx->device));
+    rc = run_ls(j, ls, ms, generation);
+    if (rc) return rc;
+    HIPCHK(j->ctx, hipStreamSynchronize(j->ctx->stream));
+    return ADGPU_OK;
+}
+
+int adgpu_job_kernel_sum_and_swap(adgpu_job_t j) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    swap_bufs(j);
+    j->generation++;
+    return ADGPU_OK;
+}
+
+int adgpu_job_generation(adgpu_job_t j) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    const LigandSet* ls; const MapSet* ms;
+    int rc = job_lookup(j, &ls, &ms);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = run_gen(j, ls, ms))) return rc;
+    bool had_ls = ls_method_for(j, j->generation) != ADGPU_LS_NONE;
+    if ((rc = run_ls(j, ls, ms, j->generation))) return rc;
+    HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    accumulate_times(j, had_ls);
+    swap_bufs(j);
+    j->generation++;
+    return ADGPU_OK;
+}
+
+// check_progress (host/src/performdocking.cpp.Cuda:908-942)
+static double progress(const adgpu_job_t j, int64_t* total) {
+    int64_t t = 0;
+    for (int r = 0; r < j->sp.num_runs; ++r) t += j->h_evals_runs[r];
+    *total = t;
+    double ep = (double)t / (double)j->sp.num_runs / (double)j->sp.max_evals * 100.0;
+    double gp = (double)j->generation / (double)j->sp.max_generations * 100.0;
+    return ep > gp ? ep : gp;
+}
+
+int adgpu_job_run(adgpu_job_t j, int64_t* out_gens, int64_t* out_total) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    const LigandSet* ls; const MapSet* ms;
+    int rc = job_lookup(j, &ls, &ms);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(j->h_evals_runs, j->b.evals_runs, j->sp.num_runs * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t total = 0;
+    while (progress(j, &total) < 100.0) {
+        if ((rc = run_gen(j, ls, ms))) return rc;
+        bool had_ls = ls_method_for(j, j->generation) != ADGPU_LS_NONE;
+        if ((rc = run_ls(j, ls, ms, j->generation))) return rc;
+        HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
+        HIPCHK(c, hipMemcpyAsync(j->h_evals_runs, j->b.evals_runs, j->sp.num_runs * sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        accumulate_times(j, had_ls);
+        swap_bufs(j);
+        j->generation++;
+    }
+    if (out_gens) *out_gens = j->generation;
+    if (out_total) *out_total = total;
+    return ADGPU_OK;
+}
+
+int adgpu_job_download(adgpu_job_t j, float* pop, float* ene, int32_t* evals_runs, uint32_t* prng) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t ne = (size_t)j->sp.num_runs * j->sp.pop_size;
+    if (pop) HIPCHK(c, hipMemcpy(pop, j->b.pop_cur, ne * kGenes * sizeof(float), hipMemcpyDeviceToHost));
+    if (ene) HIPCHK(c, hipMemcpy(ene, j->b.ene_cur, ne * sizeof(float), hipMemcpyDeviceToHost));
+    if (evals_runs) HIPCHK(c, hipMemcpy(evals_runs, j->b.evals_runs, j->sp.num_runs * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (prng) HIPCHK(c, hipMemcpy(prng, j->b.prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return ADGPU_OK;
+}
+
+int adgpu_job_upload(adgpu_job_t j, const float* pop, const float* ene, const uint32_t* prng) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t ne = (size_t)j->sp.num_runs * j->sp.pop_size;
+    if (pop) HIPCHK(c, hipMemcpy(j->b.pop_cur, pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
+    if (ene) HIPCHK(c, hipMemcpy(j->b.ene_cur, ene, ne * sizeof(float), hipMemcpyHostToDevice));
+    if (prng) HIPCHK(c, hipMemcpy(j->b.prng, prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return ADGPU_OK;
+}
+
+int adgpu_job_kernel_time(adgpu_job_t j, int cls, double* avg_ms, int64_t* launches) {
+    if (!j || cls < 0 || cls > 1) return ADGPU_ERR_ARG;
+    if (avg_ms) *avg_ms = j->kernel_n[cls] ? j->kernel_ms[cls] / (double)j->kernel_n[cls] : 0.0;
+    if (launches) *launches = j->kernel_n[cls];
+    return ADGPU_OK;
+}
+
+}  // extern "C"

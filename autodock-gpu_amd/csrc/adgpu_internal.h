@@ -1,0 +1,87 @@
+// adgpu_internal.h -- device data layout shared by the HIP kernels and the
+// host side of the library.  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/adgpu.h"
+
+namespace adgpu {
+
+constexpr int kWave = 64;            // CDNA wavefront width; one wave per individual
+constexpr int kWavesPerGroup = 4;    // individuals per workgroup (they share staged ligand tables)
+constexpr int kGenes = ADGPU_GENOTYPE_STRIDE;
+
+// One intramolecular pair, precomputed on the host from the reference's
+// per-type tables (the lookups of calcenergy.cl:397-536 folded per pair).
+struct __attribute__((aligned(16))) PairRec {
+    uint32_t ij;      // atom1 | atom2 << 16
+    uint32_t flags;   // m | n << 8 | (CG-G0 bond) << 16 | (fast 12-6 / 12-10 form) << 17
+    float ac;         // VWpars_AC of the type pair
+    float bd;         // VWpars_BD
+    float reqm;       // reqm_AB (optimum distance)
+    float cq;         // coeff_elec * q1 * q2 (same operation order as the reference)
+    float dsol;       // (S1 + qasp|q1|) V2 + (S2 + qasp|q2|) V1
+    float pad;
+};
+
+// Everything the kernels need about one ligand, in one device allocation.
+// Offsets are in bytes from the start of the block.
+struct LigandDev {
+    int32_t num_atoms, true_ligand_atoms, num_rotbonds, num_genes;
+    int32_t num_pairs, nchunk, pad0, pad1;
+    float coeff_elec, coeff_desolv, qasp, half_smooth;
+    float elec_min_distance, pad2, pad3, pad4;
+    uint32_t off_atom_ref;    // float4 [N]  x, y, z (grid units), charge
+    uint32_t off_atom_info;   // int4   [N]  map index, type, ignore_inter, 0
+    uint32_t off_atom_mask;   // uint2  [N]  bit t = moves with rotatable bond t
+    uint32_t off_tor_unit;    // float4 [T]  unit vector
+    uint32_t off_tor_move;    // float4 [T]  moving vector
+    uint32_t off_tor_atoms;   // int2   [T]  bond atoms
+    uint32_t off_pairs;       // PairRec[P]
+    uint32_t total_bytes;
+};
+
+struct GridDev {
+    int32_t gx, gy, gz, nmaps;        // nmaps = type maps + 2
+    uint32_t g1, g2, g3, format;      // g1 = gx, g2 = gx*gy, g3 = gx*gy*gz
+    float spacing, inv_spacing, pad0, pad1;
+    const void* data;                 // [nmaps][g3] points of 4 values (float4 or half4/bf16x4)
+};
+
+// Search parameters scaled as the device sees them
+// (host/src/performdocking.cpp.Cuda:427-463).
+struct SearchDev {
+    int32_t num_runs, pop_size, num_genes, num_ls;
+    float tournament_rate, crossover_rate, mutation_rate;   // in [0,1]
+    float abs_max_dmov, abs_max_dang, lsearch_rate;        // lsearch_rate in percent
+    float rho_lower_bound, base_dmov_mul_sqrt3, base_dang_mul_sqrt3;
+    int32_t cons_limit, max_iters, ls_method;
+    float adam_beta1, adam_beta2, adam_epsilon, pad;
+};
+
+struct JobBuffers {
+    float* pop_cur;
+    float* pop_next;
+    float* ene_cur;
+    float* ene_next;
+    int32_t* evals_new;
+    int32_t* evals_runs;
+    uint32_t* prng;
+};
+
+// Launchers (adgpu_kernels.hip).
+hipError_t launch_eval_poses(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
+                             const GridDev& grid, int mode, const float* genotypes, int n,
+                             float* energies, float* grads);
+hipError_t launch_initpop(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
+                          const GridDev& grid, const SearchDev& sp, const JobBuffers& b);
+hipError_t launch_gen(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
+                      const GridDev& grid, const SearchDev& sp, const JobBuffers& b);
+hipError_t launch_ls(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
+                     const GridDev& grid, const SearchDev& sp, const JobBuffers& b, int method);
+hipError_t launch_sum_evals(hipStream_t s, const SearchDev& sp, const JobBuffers& b);
+size_t shared_bytes(const LigandDev& lig_host, bool pairs_in_lds);
+
+}  // namespace adgpu

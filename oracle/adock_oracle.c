@@ -605,6 +605,10 @@ float orc_randf(uint32_t* s) { return ((float)orc_rand(s) / 4294967296.0f) * 0.9
 
 /* map_angle (auxiliary_genetic.cl:72-85) */
 static inline void map_angle(float* a) {
+    /* values the loop cannot reduce (|a| >= 1e6, inf) go through fmodf first;
+     * the reference would loop forever on them */
+    if (!(fabsf(*a) < 1.0e6f)) *a = fmodf(*a, 360.0f);
+    if (isnan(*a) || isinf(*a)) return;
     while (*a >= 360.0f) *a -= 360.0f;
     while (*a < 0.0f) *a += 360.0f;
 }
@@ -732,7 +736,8 @@ static int ls_pick(orc_job* j, int r, int le, uint32_t* st0) {
     int ent = le;
     if (ent == 0 && 100.0f * orc_randf(st0) > j->sp.lsearch_rate) ent = nls;
     (void)r;
-    return ent;
+    /* the reference may read entity pop_size (see the kernels); clamp */
+    return ent < j->sp.pop_size ? ent : j->sp.pop_size - 1;
 }
 
 /* perform_LS, Solis-Wets with SWAT3 (kernel3.cl:30-436) */
