@@ -1,0 +1,112 @@
+"""Shared test helpers: synthetic cases, golden fixtures, and the parity
+tolerances used across the suite."""
+from __future__ import annotations
+
+import os
+from types import SimpleNamespace
+
+import numpy as np
+
+from adgpu import synth
+from adgpu._abi import CGrid, CLigand, energy_params
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# Per-pose energy parity bar: |E_gpu - E_oracle| <= ENERGY_RTOL * max(1, S)
+# where S = sum of |energy terms| of the pose (the scale FP32 rounding error of
+# a sum is proportional to).
+ENERGY_RTOL = 1e-4
+# Gradient genes are int32 fixed point (1/1024 per term); a per-term rounding
+# flip between two FP32 evaluations moves a gene by one quantum.
+QUANTUM = 1.0 / 1024.0
+
+
+def synthetic_case(n_atoms, n_tors, lig_seed, map_seed, npts=60, spacing=0.375, n_wells=200):
+    lig = synth.make_ligand(n_atoms, n_tors, seed=lig_seed)
+    maps = synth.make_maps(lig.type_names, npts=npts, spacing=spacing, seed=map_seed, n_wells=n_wells)
+    T = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+    packed = maps.packed()
+    ep = energy_params()
+    cg = CGrid(packed, maps.size, maps.num_map_atypes, maps.spacing)
+    cl = CLigand(T)
+    orc = O.Oracle(cg, cl, ep)
+    return SimpleNamespace(lig=lig, maps=maps, tables=T, packed=packed, ep=ep, cg=cg, cl=cl, orc=orc,
+                           ng=T.num_genes, size=maps.size, spacing=maps.spacing)
+
+
+def poses_for(case, n, seed, frac_central=0.5, spread=6.0):
+    """Random poses (uniform translation, quaternion orientation, torsions in
+    [-180,180)); a fraction is moved near the grid centre so most atoms are
+    inside the maps."""
+    g = synth.random_poses(n, case.ng, case.size, seed=seed)
+    k = int(n * frac_central)
+    rng = np.random.default_rng(seed + 1)
+    g[:k, 0:3] = (np.array(case.size, np.float32) * 0.5 + rng.uniform(-spread, spread, size=(k, 3))).astype(np.float32)
+    return g
+
+
+def gene_quantum(case, gene):
+    """Value of one fixed-point quantum of a gene's gradient."""
+    conv = np.float32(case.spacing) * np.float32(case.spacing) * np.float32(1.0) / (np.float32(0.0174533) ** 2)
+    return QUANTUM if gene < 3 else QUANTUM * float(conv)
+
+
+def load_ref_fixture(name):
+    z = np.load(os.path.join(GOLDEN, "ref_%s.npz" % name))
+    return {k: z[k] for k in z.files}
+
+
+def tables_from_fixture(f):
+    keys = ("charges", "atom_types", "atom_map", "ignore_inter", "ref_coords", "rotbond_moving", "rotbond_unit",
+            "rotbond_atoms", "atom_rotbonds", "pairs", "atype_reqm", "vw_exp", "reqm_ab", "vw_ac", "vw_bd", "ds_s",
+            "ds_v")
+    t = SimpleNamespace(**{k: f[k] for k in keys})
+    t.num_atoms = int(f["num_atoms"])
+    t.true_ligand_atoms = int(f["true_ligand_atoms"])
+    t.num_rotbonds = int(f["num_rotbonds"])
+    t.num_atypes = int(f["num_atypes"])
+    t.num_genes = t.num_rotbonds + 6
+    return t
+
+
+def load_dlg(name):
+    z = np.load(os.path.join(GOLDEN, "dlg_%s.npz" % name))
+    return {k: z[k] for k in z.files}
+
+
+def single_atom_tables(type_index, charge):
+    """A one-atom ligand (no torsions, no pairs) whose energy at translation
+    genes (x,y,z) and zero orientation is that atom's intermolecular energy."""
+    return SimpleNamespace(
+        num_atoms=1, true_ligand_atoms=1, num_rotbonds=0, num_atypes=1, num_genes=6,
+        charges=np.array([charge], np.float32), atom_types=np.zeros(1, np.int32),
+        atom_map=np.array([type_index], np.int32), ignore_inter=np.zeros(1, np.uint8),
+        ref_coords=np.zeros(3, np.float32), rotbond_moving=np.zeros(0, np.float32),
+        rotbond_unit=np.zeros(0, np.float32), rotbond_atoms=np.zeros(0, np.int32),
+        atom_rotbonds=np.zeros(0, np.uint8), pairs=np.zeros(0, np.int32), atype_reqm=np.array([3], np.uint32),
+        vw_exp=np.array([(12 << 8) + 6], np.uint16), reqm_ab=np.ones(1, np.float32), vw_ac=np.zeros(1, np.float32),
+        vw_bd=np.zeros(1, np.float32), ds_s=np.zeros(1, np.float32), ds_v=np.zeros(1, np.float32))
+
+
+def rigid_tables(types_idx, charges, xyz_grid):
+    """A rigid multi-atom ligand without intramolecular pairs: its energy at
+    identity orientation is the sum of the atoms' intermolecular energies."""
+    n = len(charges)
+    c = np.asarray(xyz_grid, np.float64)
+    centre = c.mean(axis=0)
+    t = single_atom_tables(0, 0.0)
+    t.num_atoms = n
+    t.true_ligand_atoms = n
+    t.charges = np.asarray(charges, np.float32)
+    t.atom_types = np.zeros(n, np.int32)
+    t.atom_map = np.asarray(types_idx, np.int32)
+    t.ignore_inter = np.zeros(n, np.uint8)
+    t.ref_coords = (c - centre).astype(np.float32).reshape(-1)
+    return t, centre
+
+
+def angle_diff(a, b):
+    """Difference of angles in degrees modulo 360, in [0, 180]."""
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) % 360.0
+    return np.minimum(d, 360.0 - d)

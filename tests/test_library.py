@@ -1,0 +1,45 @@
+"""The C-ABI library (no GPU needed): it loads and exports every entry point
+include/adgpu.h declares; the oracle library exports what its header declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(adgpu_\w+|orc_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_symbols_exported():
+    from adgpu import _abi
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.fail("libadgpu_hip.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    names = _declared(os.path.join(REPO, "include", "adgpu.h"))
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_abi.EXPORTED)
+    lib.adgpu_version.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.adgpu_version()
+
+
+def test_null_arguments_are_rejected_without_gpu():
+    from adgpu import _abi
+    lib = _abi.load_library()
+    assert lib.adgpu_open(0, None) == 1                      # ADGPU_ERR_ARG
+    assert lib.adgpu_close(None) == 1
+    assert lib.adgpu_job_destroy(None) == 1
+    assert lib.adgpu_last_error(None) == b"null context"
+
+
+def test_oracle_symbols():
+    import oracle
+    lib = oracle.lib()
+    names = _declared(os.path.join(REPO, "oracle", "adock_oracle.h"))
+    assert not [n for n in names if not hasattr(lib, n)]
