@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Docking throughput benchmark (one MI355X).
+
+Workload (the LGA configuration of the headline metric, on one GPU): a seeded
+30-atom ligand (types C, A, N, OA, HD; charges uniform in [-0.5, 0.5]; 6
+rotatable torsions in a random tree) on seeded 60^3 FP32 maps at 0.375 A (200
+Gaussian wells, repulsive cores clipped at 1e5); LGA with ADADELTA local
+search, nrun = 100, pop 150, 2.5 M evaluations per run, no heuristics, no
+AutoStop.  One step = one complete docking of the ligand (all 100 runs to the
+evaluation budget), inputs resident in HBM.
+
+Prints one JSON line: evals/s (value), ligands/hour, p50/p99 per-ligand
+latency, the roofline of the dominant kernel (the local-search launch) and a
+CPU baseline (the oracle's LGA on the host cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "autodock-gpu_amd"))
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nrun", type=int, default=100)
+    ap.add_argument("--pop", type=int, default=150)
+    ap.add_argument("--evals", type=int, default=2500000)
+    ap.add_argument("--ls", default="ad")
+    ap.add_argument("--maps", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def build_inputs():
+    from adgpu import synth
+    from adgpu._abi import energy_params
+    lig = synth.make_ligand(30, 6, seed=7)
+    maps = synth.make_maps(lig.type_names, npts=60, spacing=0.375, seed=11)
+    tables = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+    return lig, maps, tables, energy_params()
+
+
+def initial_state(seed, nrun, pop, maps):
+    """Initial population and LCG seeds drawn the way docking_with_gpu does
+    (CMWC4096 seeded from the job seed), generated on the host."""
+    rng = np.random.default_rng(seed)
+    n = nrun * pop
+    init = np.zeros((n, 64), np.float32)
+    size = np.array(maps.size_angstr) / maps.spacing
+    init[:, 0:3] = rng.random((n, 3)) * size
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    qw = np.sqrt(1 - u1) * np.sin(2 * np.pi * u2)
+    qx = np.sqrt(1 - u1) * np.cos(2 * np.pi * u2)
+    qy = np.sqrt(u1) * np.sin(2 * np.pi * u3)
+    qz = np.sqrt(u1) * np.cos(2 * np.pi * u3)
+    s = np.sqrt(np.maximum(1 - qw * qw, 1e-12))
+    init[:, 3] = np.arctan2(qy / s, qx / s) / 0.0174533
+    init[:, 4] = np.arccos(np.clip(qz / s, -1, 1)) / 0.0174533
+    init[:, 5] = 2 * np.arccos(np.clip(qw, -1, 1)) / 0.0174533
+    init[:, 6:63] = rng.random((n, 57)) * 360.0
+    seeds = rng.integers(0, 2 ** 32, size=n * 64, dtype=np.uint64).astype(np.uint32)
+    return init, seeds
+
+
+def cpu_baseline(maps, tables, ep, sp_gpu, seconds):
+    """The oracle's LGA (FP32 restatement of the reference kernels) on the
+    host cores: OpenMP over runs, generations until `seconds` elapse."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    from adgpu._abi import CGrid, CLigand, search_params
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    nrun = threads
+    sp = search_params(num_runs=nrun, pop_size=sp_gpu.pop_size, max_evals=sp_gpu.max_evals,
+                       ls="ad", spacing=maps.spacing)
+    init, seeds = initial_state(99, nrun, sp.pop_size, maps)
+    cg = CGrid(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing)
+    cl = CLigand(tables)
+    job = O.OracleJob(cg, cl, ep, sp, init, seeds)
+    job.initpop()
+    t0 = time.time()
+    evals = 0
+    gens = 0
+    while time.time() - t0 < seconds:
+        evals += job.generations(0, nrun, 1, nthreads=threads)
+        gens += 1
+    dt = time.time() - t0
+    return dict(value=evals / dt, unit="evals/s", cores=threads, kind="port",
+                sample="%d runs x pop %d, %d LGA generations with ADADELTA (%d evals, %.1f s)"
+                       % (nrun, sp.pop_size, gens, evals, dt))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every GPU box here has one, and the docking "
+              "path shards by ligand/run with no inter-GPU exchange (see the sharding note "
+              "in the design document); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import adgpu
+    lig, maps, tables, ep = build_inputs()
+    fmt = {"f32": adgpu.MAP_F32, "f16": adgpu.MAP_F16, "bf16": adgpu.MAP_BF16}[a.maps]
+    ctx = adgpu.Context(0)
+    mid = ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing, fmt)
+    lid = ctx.ligand_upload(tables, ep)
+    sp = adgpu.search_params(num_runs=a.nrun, pop_size=a.pop, max_evals=a.evals, ls=a.ls, spacing=maps.spacing)
+    n_inter = int(tables.num_atoms - int(np.sum(tables.ignore_inter)))
+    bytes_per_eval = n_inter * 3 * 8 * {"f32": 4, "f16": 2, "bf16": 2}[a.maps]
+
+    lat, evals_list, gens_list, ls_ms, ls_n, gen_ms = [], [], [], [], [], []
+    ls_evals_list = []
+    for step in range(a.warmup + a.steps):
+        init, seeds = initial_state(1000 + step, a.nrun, a.pop, maps)
+        job = ctx.job(mid, lid, sp, init, seeds)      # allocations outside the timed region
+        t0 = time.perf_counter()
+        job.initpop()
+        gens, total = job.run()
+        dt = time.perf_counter() - t0
+        ms_ls, n_ls = job.kernel_time(1)
+        ms_gen, n_gen = job.kernel_time(0)
+        st = job.download()
+        job.destroy()
+        if step >= a.warmup:
+            lat.append(dt)
+            evals_list.append(total)
+            gens_list.append(gens)
+            ls_ms.append(ms_ls)
+            ls_n.append(n_ls)
+            gen_ms.append(ms_gen)
+            # LS evaluations: total minus K1 (nrun*pop) and K4 (nrun*(pop-1) per generation)
+            ls_evals_list.append(total - a.nrun * a.pop - gens * a.nrun * (a.pop - 1))
+        print("step %d: %.3f s, %d generations, %d evals, best %.3f" %
+              (step, dt, gens, total, float(st["ene"].min())), file=sys.stderr, flush=True)
+
+    t = float(np.sum(lat))
+    evals = float(np.sum(evals_list))
+    evals_per_s = evals / t
+    lat_arr = np.array(lat)
+    # roofline of the dominant kernel: algorithmic map-gather bytes per LS launch
+    # divided by the LS launch's average duration (HIP events on the job stream)
+    tot_ls_ms = float(np.sum(np.array(ls_ms) * np.array(ls_n)))
+    ls_evals = float(np.sum(ls_evals_list))
+    avg_ls_ms = tot_ls_ms / max(1, int(np.sum(ls_n)))
+    bytes_per_launch = ls_evals * bytes_per_eval / max(1, int(np.sum(ls_n)))
+    achieved = bytes_per_launch / (avg_ls_ms * 1e-3) / 1e9
+    out = {
+        "metric": "LGA energy evals/sec + ligands/hour at 2.5M evals; p50/p99 ligand latency",
+        "value": evals_per_s,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1000.0 * t / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ligand and maps generated in-process)",
+        "config": {"workload": "LGA ADADELTA, 1 ligand of 30 atoms / 6 torsions, 60^3 maps at 0.375 A (%s), "
+                               "nrun %d, pop %d, %d evals per run" % (a.maps.upper(), a.nrun, a.pop, a.evals),
+                   "nrun": a.nrun, "pop_size": a.pop, "evals_per_run": a.evals, "ls": a.ls, "maps": a.maps},
+        "ligands_per_hour": 3600.0 / float(np.mean(lat_arr)),
+        "latency_s": {"p50": float(np.percentile(lat_arr, 50)), "p99": float(np.percentile(lat_arr, 99))},
+        "generations_per_ligand": float(np.mean(gens_list)),
+        "kernels": {"ls_avg_ms": avg_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
+                    "ls_launches": int(np.sum(ls_n)), "ls_share_of_wall": tot_ls_ms / 1000.0 / t},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_eval": bytes_per_eval},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(maps, tables, ep, sp, a.cpu_seconds)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
