@@ -78,7 +78,11 @@ int fail(adgpu_context* c, int code, const std::string& msg) {
             return fail((ctx), ADGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// FP16 storage saturates at the largest finite half (maps clip repulsive
+// cores at 1e5, above the FP16 range).
 uint16_t f32_to_f16(float f) {
+    if (f > 65504.0f) f = 65504.0f;
+    if (f < -65504.0f) f = -65504.0f;
     _Float16 h = (_Float16)f;
     uint16_t u;
     std::memcpy(&u, &h, 2);

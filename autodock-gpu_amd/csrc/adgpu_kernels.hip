@@ -851,6 +851,9 @@ __global__ __launch_bounds__(256) void k_gen(Launch a) {
         child = v0;
     }
     if (lane < NG) {
+        // separate rounding of every operation, as the reference source
+        // writes it (no fused multiply-add): the new genes are bit-exact
+#pragma clang fp contract(off)
         if (randf(&state) < a.S.mutation_rate) {
             if (lane < 3) child += a.S.abs_max_dmov * (2 * randf(&state) - 1);
             else { child += a.S.abs_max_dang * (2 * randf(&state) - 1); map_angle(child); }

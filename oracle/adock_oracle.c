@@ -337,8 +337,43 @@ static void torque_to_rotgenes(V4 torque, const Pose* P, int use_fmod_pi2, R out
 /* ------------------------------------------------------------------------- */
 /* gpu_calc_energrad (calcMergedEneGra.cl:46-936)                             */
 /* ------------------------------------------------------------------------- */
+static R energrad_core(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                       const float* genotype, int quantize, R* grad, R* atom_grad);
+
 R SFX(orc_energrad)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
                     const float* genotype, int quantize, R* grad) {
+    return energrad_core(g, L, ep, genotype, quantize, grad, NULL);
+}
+
+/* Per-atom gradient dE/dx (kcal/mol/A) and energy-sensitivity scale
+ * sum_a |dE/dx_a| * |x_a| (x in A from the grid origin): the first-order change
+ * of E under a relative coordinate perturbation. */
+R SFX(orc_atom_grad)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                     const float* genotype, R* atom_grad, R* sensitivity) {
+    R grad[ADGPU_GENOTYPE_STRIDE];
+    R ag[3 * ADGPU_MAX_ATOMS];
+    R e = energrad_core(g, L, ep, genotype, 0, grad, ag);
+    if (atom_grad) memcpy(atom_grad, ag, sizeof(R) * 3 * L->num_atoms);
+    if (sensitivity) {
+        V4 c[ADGPU_MAX_ATOMS];
+        Pose P;
+        pose_coords(L, genotype, c, &P);
+        R s = 0;
+        for (int a = 0; a < L->num_atoms; ++a) {
+            /* an atom outside the maps carries the constant 2^24 penalty,
+             * whose "gradient" entries say nothing about sensitivity */
+            if (!L->ignore_inter[a] && out_of_grid(g, c[a].x, c[a].y, c[a].z)) continue;
+            R fn = sqrt(ag[3 * a] * ag[3 * a] + ag[3 * a + 1] * ag[3 * a + 1] + ag[3 * a + 2] * ag[3 * a + 2]);
+            R xn = sqrt(c[a].x * c[a].x + c[a].y * c[a].y + c[a].z * c[a].z) * (R)g->spacing;
+            s += fn * xn;
+        }
+        *sensitivity = s;
+    }
+    return e;
+}
+
+static R energrad_core(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                       const float* genotype, int quantize, R* grad, R* atom_grad) {
     V4 c[ADGPU_MAX_ATOMS];
     Pose P;
     pose_coords(L, genotype, c, &P);
@@ -412,6 +447,7 @@ R SFX(orc_energrad)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_ener
     for (int a = 0; a < N; ++a) {
         if (quantize) F[a] = v4(ONEOVERTERMSCALE * (R)gi[a][0], ONEOVERTERMSCALE * (R)gi[a][1], ONEOVERTERMSCALE * (R)gi[a][2], 0);
         else F[a] = v4(gf[a][0], gf[a][1], gf[a][2], 0);
+        if (atom_grad) { atom_grad[3 * a] = F[a].x; atom_grad[3 * a + 1] = F[a].y; atom_grad[3 * a + 2] = F[a].z; }
     }
     R gg[ADGPU_GENOTYPE_STRIDE];
     int32_t ig[ADGPU_GENOTYPE_STRIDE];

@@ -40,6 +40,14 @@ float  orc_energrad_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_
 double orc_energrad_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
                         const float* genotype, int quantize, double* grad);
 
+/* Per-atom dE/dx (3*num_atoms, kcal/mol/A, unquantized) and the energy's
+ * sensitivity to coordinate rounding, sum_a |dE/dx_a| |x_a| (A): FP32 pose
+ * arithmetic perturbs x_a by ~1e-7 |x_a|, which moves E by ~1e-7 times this. */
+float  orc_atom_grad_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                         const float* genotype, float* atom_grad, float* sensitivity);
+double orc_atom_grad_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                         const float* genotype, double* atom_grad, double* sensitivity);
+
 /* Float gradient used by FIRE's first step.  calcgradient.cl:109-1003. */
 void   orc_gradient_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
                         const float* genotype, float* grad);

@@ -49,6 +49,8 @@ def lib():
         L.orc_energrad_f32.argtypes = [gp, lp, ep, _fp, C.c_int, _fp]
         L.orc_energrad_f64.restype = C.c_double
         L.orc_energrad_f64.argtypes = [gp, lp, ep, _fp, C.c_int, _dp]
+        L.orc_atom_grad_f64.restype = C.c_double
+        L.orc_atom_grad_f64.argtypes = [gp, lp, ep, _fp, _dp, _dp]
         L.orc_gradient_f32.argtypes = [gp, lp, ep, _fp, _fp]
         L.orc_gradient_f64.argtypes = [gp, lp, ep, _fp, _dp]
         L.orc_coords_f32.argtypes = [lp, _fp, _fp]
@@ -112,6 +114,13 @@ class Oracle:
         out = np.zeros(64, np.float64)
         e = lib().orc_energrad_f64(*self._args(), g.ctypes.data_as(_fp), int(quantize), out.ctypes.data_as(_dp))
         return float(e), out[: self.ng].copy()
+
+    def sensitivity(self, geno):
+        """sum_a |dE/dx_a| |x_a| of the pose (FP64)."""
+        g = _f(geno)
+        s = C.c_double()
+        lib().orc_atom_grad_f64(*self._args(), g.ctypes.data_as(_fp), None, C.byref(s))
+        return float(s.value)
 
     def gradient32(self, geno):
         g = _f(geno)
