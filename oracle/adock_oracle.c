@@ -360,10 +360,11 @@ R SFX(orc_atom_grad)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_ene
         pose_coords(L, genotype, c, &P);
         R s = 0;
         for (int a = 0; a < L->num_atoms; ++a) {
-            /* an atom outside the maps carries the constant 2^24 penalty,
-             * whose "gradient" entries say nothing about sensitivity */
-            if (!L->ignore_inter[a] && out_of_grid(g, c[a].x, c[a].y, c[a].z)) continue;
-            R fn = sqrt(ag[3 * a] * ag[3 * a] + ag[3 * a + 1] * ag[3 * a + 1] + ag[3 * a + 2] * ag[3 * a + 2]);
+            /* an atom outside the maps carries the constant 2^24 penalty in
+             * its gradient entries; only its pair forces measure sensitivity */
+            R off = (!L->ignore_inter[a] && out_of_grid(g, c[a].x, c[a].y, c[a].z)) ? (R)16777216.0f : (R)0;
+            R fx = ag[3 * a] - off, fy = ag[3 * a + 1] - off, fz = ag[3 * a + 2] - off;
+            R fn = sqrt(fx * fx + fy * fy + fz * fz);
             R xn = sqrt(c[a].x * c[a].x + c[a].y * c[a].y + c[a].z * c[a].z) * (R)g->spacing;
             s += fn * xn;
         }
