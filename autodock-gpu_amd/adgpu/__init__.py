@@ -52,6 +52,10 @@ class Context:
             msg = self.lib.adgpu_last_error(self.h)
             raise AdgpuError("%s failed (status %d): %s" % (what, rc, msg.decode() if msg else ""))
 
+    def set_team_width(self, width: int):
+        """0 = per ligand, 32 = two individuals per wavefront, 64 = one."""
+        self._check(self.lib.adgpu_set_team_width(self.h, int(width)), "adgpu_set_team_width")
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.adgpu_close(self.h)

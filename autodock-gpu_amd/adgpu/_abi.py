@@ -157,6 +157,7 @@ def load_library(path: str | None = None):
     lib.adgpu_last_error.argtypes = [vp]
     lib.adgpu_open.argtypes = [C.c_int, C.POINTER(vp)]
     lib.adgpu_close.argtypes = [vp]
+    lib.adgpu_set_team_width.argtypes = [vp, C.c_int]
     lib.adgpu_maps_upload.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(C.c_int)]
     lib.adgpu_maps_release.argtypes = [vp, C.c_int]
     lib.adgpu_ligand_upload.argtypes = [vp, C.POINTER(Ligand), C.POINTER(EnergyParams), C.POINTER(C.c_int)]
@@ -177,7 +178,7 @@ def load_library(path: str | None = None):
     return lib
 
 
-EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "adgpu_maps_upload",
+EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "adgpu_set_team_width", "adgpu_maps_upload",
             "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses",
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",
             "adgpu_job_kernel_ls", "adgpu_job_kernel_sum_and_swap", "adgpu_job_generation", "adgpu_job_run",

@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
@@ -15,7 +16,9 @@ REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(HERE, "libadgpu_hip.so")
 
-SOURCES = ["adgpu_kernels.hip", "adgpu_host.cpp"]
+# one translation unit per kernel family, compiled in parallel
+SOURCES = ["k_ls_ad.hip", "k_ls_sw.hip", "k_ls_adam.hip", "k_ls_fire.hip", "k_ga.hip", "k_eval.hip",
+           "adgpu_host.cpp"]
 ARCH = os.environ.get("ADGPU_OFFLOAD_ARCH", "gfx950")
 
 
@@ -33,19 +36,23 @@ def flags():
             "-I", os.path.join(REPO, "include"), "-Wno-unused-command-line-argument"]
 
 
-def build(verbose: bool = False) -> str:
-    objs = []
+def _compile(src: str, tmp: str, verbose: bool) -> str:
+    obj = os.path.join(tmp, src + ".o")
+    cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".hip"):
+        cmd[1:1] = ["-x", "hip"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
     tmp = os.path.join(PKG, "_build")
     os.makedirs(tmp, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(tmp, src + ".o")
-        cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".hip"):
-            cmd[1:1] = ["-x", "hip"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    jobs = jobs or max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1), 16))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, tmp, verbose), SOURCES))
     tmp_out = OUT + ".tmp"
     cmd = [hipcc(), "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp_out] + objs
     if verbose:

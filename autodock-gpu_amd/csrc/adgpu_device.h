@@ -1,0 +1,966 @@
+// adgpu_device.h -- gfx950 device code of the docking hot path (included by
+// the kernel translation units k_*.hip; each instantiates its own kernels).
+//
+// Execution model (CDNA, 64-wide wavefronts):
+//   * a "team" of TW lanes (TW = 32 or 64) evaluates one individual, so a
+//     wavefront carries 64/TW individuals; a 256-thread workgroup stages the
+//     ligand tables (pair records, torsion vectors) into LDS once and every
+//     team of the workgroup shares them;
+//   * team lane l owns atoms l, l+TW, ... (NC chunks) and walks each atom's
+//     own torsion chain, so pose -> coordinates needs no barrier (the
+//     reference's rotation list is a barrier per cycle, calcenergy.cl:232-275);
+//   * team lane l owns genes l and l+TW (when TW = 32) for a whole local
+//     search, so minimiser updates are register code; each gene lane also
+//     evaluates the sine/cosine of its own angle once per evaluation;
+//   * pairs are strided over the team; pair forces go to per-atom int32
+//     fixed-point accumulators in LDS through ds_add (order independent);
+//   * energies, forces and torques are summed with DPP row reductions plus
+//     readlane, which leaves identical bits in every lane of the team;
+//   * local searches run as persistent kernels: a team that finishes its
+//     entity takes the next one from a global counter, so teams of one
+//     wavefront never wait for each other and uneven searches balance out.
+// The arithmetic of every device function follows the reference file:line
+// given at its definition (ccsb-scripps/AutoDock-GPU v1.5, device/ = OpenCL).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "adgpu_internal.h"
+
+namespace adgpu {
+
+#define DEV __device__ __forceinline__
+// 256-thread workgroups; at least 4 wavefronts per SIMD (<= 128 VGPRs)
+#define ADGPU_GROUP_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+
+// common/calcenergy_basic.h
+constexpr float kDegToRad = 0.0174533f;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 2.0f * kPi;
+constexpr float kTermScale = 1024.0f;
+constexpr float kOneOverTermScale = 1.0f / 1024.0f;
+constexpr float kMaxTerm = 8192.0f;
+
+// ---------------------------------------------------------------------------
+// wavefront / team primitives
+// ---------------------------------------------------------------------------
+DEV int lane_id() { return (int)__lane_id(); }
+
+// Orders LDS accesses of the lanes of one wavefront (no workgroup barrier).
+DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int CTRL>
+DEV float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+DEV float rdl(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+DEV int rdl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <int TW>
+struct Team {
+    static_assert(TW == 32 || TW == 64, "team width");
+    static constexpr int kPerWave = 64 / TW;
+    static constexpr int kGPL = 64 / TW;          // genes per lane
+    static constexpr int kPerGroup = 256 / TW;    // teams per 256-thread workgroup
+    DEV static int lane() { return lane_id() & (TW - 1); }
+    DEV static int slot() { return lane_id() / TW; }
+    // Sum over the team; every lane of the team receives the same bits.
+    // Rows of 16 lanes: xor 1, xor 2 (quad_perm), half-row mirror, row mirror;
+    // then the row sums are combined through readlane.
+    DEV static float sum(float v) {
+        v += dpp<0xB1>(v);
+        v += dpp<0x4E>(v);
+        v += dpp<0x141>(v);
+        v += dpp<0x140>(v);
+        if constexpr (TW == 32) {
+            float a = rdl(v, 0) + rdl(v, 16);
+            float b = rdl(v, 32) + rdl(v, 48);
+            return lane_id() < 32 ? a : b;
+        } else {
+            return (rdl(v, 0) + rdl(v, 16)) + (rdl(v, 32) + rdl(v, 48));
+        }
+    }
+    // Value of team lane k (k uniform).
+    DEV static float bcast(float v, int k) {
+        if constexpr (TW == 32) return lane_id() < 32 ? rdl(v, k) : rdl(v, 32 + k);
+        else return rdl(v, k);
+    }
+    DEV static int bcast_i(int v, int k) {
+        if constexpr (TW == 32) return lane_id() < 32 ? rdl_i(v, k) : rdl_i(v, 32 + k);
+        else return rdl_i(v, k);
+    }
+    // True when any lane of the wavefront has p set.
+    DEV static bool any_in_wave(bool p) { return __ballot(p) != 0ull; }
+};
+
+// ---------------------------------------------------------------------------
+// scalar helpers
+// ---------------------------------------------------------------------------
+DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+DEV float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+DEV float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// fmod_pi2, calcenergy.cl:86-91
+DEV float fmod_pi2(float x) { return x - (float)((int)((1.0f / kTwoPi) * x)) * kTwoPi; }
+
+// fast_acos, calcenergy.cl:93-108
+DEV float fast_acos(float cosine) {
+    const float a = 9.78056e-05f, b = -0.00104588f, c = 0.00418716f, d = -0.00314347f;
+    const float e = 2.74084f, f = 0.370388f;
+    const float o = -(a + b + c + d);
+    float x = fabsf(cosine);
+    float x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+    float ac = (((o * x4 + a) * x3 + b) * x2 + c) * x + d +
+               e * fast_sqrt(2.0f - fast_sqrt(2.0f + 2.0f * x)) - f * fast_sqrt(2.0f - 2.0f * x);
+    return copysignf(ac, cosine) + (cosine < 0.0f ? kPi : 0.0f);
+}
+
+// float2int_round of the clamped term, calcMergedEneGra.cl:30-35 (the half is
+// moved toward zero before truncation, as the reference does).
+DEV int quant(float x) {
+    float v = fminf(kMaxTerm, fmaxf(-kMaxTerm, x));
+    int addsub = (int)(v < 0.0f) - (int)(v > 0.0f);
+    return (int)(v + 0.5f * (float)addsub);
+}
+
+// quaternion_rotate, calcenergy.cl:119-127
+DEV float3 qrot(float3 v, float4 q) {
+    float zx = (q.y * v.z - q.z * v.y) * 2.0f;
+    float zy = (q.z * v.x - q.x * v.z) * 2.0f;
+    float zz = (q.x * v.y - q.y * v.x) * 2.0f;
+    return make_float3(v.x + zx * q.w + (q.y * zz - q.z * zy),
+                       v.y + zy * q.w + (q.z * zx - q.x * zz),
+                       v.z + zz * q.w + (q.x * zy - q.y * zx));
+}
+
+// quaternion_multiply, calcenergy.cl:110-117
+DEV float4 qmul(float4 a, float4 b) {
+    return make_float4(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+                       a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+                       a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w,
+                       a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+
+// map_angle, auxiliary_genetic.cl:72-85.  Values the subtraction loop could
+// not bring into range (|a| >= 1e6, where a - 360 may round back to a, and
+// inf) are reduced with fmodf first; the reference loops forever on them.
+DEV void map_angle(float& a) {
+    if (!(fabsf(a) < 1.0e6f)) a = fmodf(a, 360.0f);
+    if (!(a == a) || isinf(a)) return;
+    while (a >= 360.0f) a -= 360.0f;
+    while (a < 0.0f) a += 360.0f;
+}
+
+// LCG, auxiliary_genetic.cl:29-67
+DEV float randf(uint32_t& s) {
+    s = 1103515245u * s + 12345u;
+    return ((float)s * (1.0f / 4294967296.0f)) * 0.999999f;
+}
+
+// ---------------------------------------------------------------------------
+// map gathers: one grid point of one map = 4 values (packed-4 layout)
+// ---------------------------------------------------------------------------
+template <int MF>
+DEV float4 load_pt(const void* base, uint32_t idx);
+
+template <>
+DEV float4 load_pt<ADGPU_MAP_F32>(const void* base, uint32_t idx) {
+    return reinterpret_cast<const float4*>(base)[idx];
+}
+template <>
+DEV float4 load_pt<ADGPU_MAP_F16>(const void* base, uint32_t idx) {
+    uint2 r = reinterpret_cast<const uint2*>(base)[idx];
+    __half2 lo = *reinterpret_cast<__half2*>(&r.x), hi = *reinterpret_cast<__half2*>(&r.y);
+    float2 a = __half22float2(lo), b = __half22float2(hi);
+    return make_float4(a.x, a.y, b.x, b.y);
+}
+template <>
+DEV float4 load_pt<ADGPU_MAP_BF16>(const void* base, uint32_t idx) {
+    uint2 r = reinterpret_cast<const uint2*>(base)[idx];
+    return make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xffff0000u),
+                       __uint_as_float(r.y << 16), __uint_as_float(r.y & 0xffff0000u));
+}
+
+// ---------------------------------------------------------------------------
+// LDS layout
+// ---------------------------------------------------------------------------
+struct SharedView {           // staged once per workgroup
+    const PairRec* pairs;     // LDS (or global when the pairs do not fit)
+    const float4* tor_unit;
+    const float4* tor_move;
+    const int2* tor_atoms;
+};
+
+struct TeamView {             // private to one team
+    float4* coords;           // [TW*NC] atom coordinates (grid units)
+    int* fx;                  // [TW*NC] int32 fixed-point pair forces
+    int* fy;
+    int* fz;
+    float4* tq;               // [T] torsion quaternions
+    float4* tax;              // [T] torsion axes (gradient)
+    float4* tref;             // [T] torsion reference atoms (gradient)
+    int* gi;                  // [T] torsion gene gradients (int fixed point)
+    float* pg;                // [12] genes 0..5, sin/cos of phi, theta, rot/2
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t shared_bytes_for(int T, int P, bool pairs_in_lds) {
+    size_t s = pairs_in_lds ? (size_t)P * sizeof(PairRec) : 0;
+    s += (size_t)T * (sizeof(float4) * 2 + sizeof(int2));
+    return align16(s);
+}
+
+__host__ __device__ inline size_t team_bytes_for(int natoms_pad, int T) {
+    return align16((size_t)natoms_pad * 16) + 3 * align16((size_t)natoms_pad * 4) + (size_t)T * 48 +
+           align16((size_t)T * 4) + 48;
+}
+
+__host__ __device__ inline size_t lds_bytes_for(int TW, int NC, int T, int P, bool pairs_in_lds) {
+    return shared_bytes_for(T, P, pairs_in_lds) + (size_t)(256 / TW) * team_bytes_for(TW * NC, T);
+}
+
+// Stages the shared tables (all threads of the workgroup) and carves the
+// team's region.  Contains the only workgroup barrier of every kernel.
+template <int TW, int NC, bool PL>
+DEV void stage(const LigandDev* L, char* lds, SharedView& sv, TeamView& tv) {
+    const char* gbase = reinterpret_cast<const char*>(L);
+    const int T = L->num_rotbonds, P = L->num_pairs;
+    char* p = lds;
+    const PairRec* gpairs = reinterpret_cast<const PairRec*>(gbase + L->off_pairs);
+    if (PL) {
+        PairRec* lp = reinterpret_cast<PairRec*>(p);
+        const uint4* src = reinterpret_cast<const uint4*>(gpairs);
+        uint4* dst = reinterpret_cast<uint4*>(lp);
+        for (int k = threadIdx.x; k < 2 * P; k += blockDim.x) dst[k] = src[k];
+        sv.pairs = lp;
+        p += (size_t)P * sizeof(PairRec);
+    } else {
+        sv.pairs = gpairs;
+    }
+    float4* tu = reinterpret_cast<float4*>(p);
+    float4* tm = tu + T;
+    int2* ta = reinterpret_cast<int2*>(tm + T);
+    const float4* gtu = reinterpret_cast<const float4*>(gbase + L->off_tor_unit);
+    const float4* gtm = reinterpret_cast<const float4*>(gbase + L->off_tor_move);
+    const int2* gta = reinterpret_cast<const int2*>(gbase + L->off_tor_atoms);
+    for (int k = threadIdx.x; k < T; k += blockDim.x) { tu[k] = gtu[k]; tm[k] = gtm[k]; ta[k] = gta[k]; }
+    sv.tor_unit = tu; sv.tor_move = tm; sv.tor_atoms = ta;
+    const int npad = TW * NC;
+    char* w = lds + shared_bytes_for(T, P, PL) + (size_t)(threadIdx.x / TW) * team_bytes_for(npad, T);
+    tv.coords = reinterpret_cast<float4*>(w);
+    w += align16((size_t)npad * 16);
+    tv.fx = reinterpret_cast<int*>(w); w += align16((size_t)npad * 4);
+    tv.fy = reinterpret_cast<int*>(w); w += align16((size_t)npad * 4);
+    tv.fz = reinterpret_cast<int*>(w); w += align16((size_t)npad * 4);
+    tv.tq = reinterpret_cast<float4*>(w);
+    tv.tax = tv.tq + T;
+    tv.tref = tv.tax + T;
+    w += (size_t)T * 48;
+    tv.gi = reinterpret_cast<int*>(w);
+    w += align16((size_t)T * 4);
+    tv.pg = reinterpret_cast<float*>(w);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// per-lane atom data, loaded once per kernel
+// ---------------------------------------------------------------------------
+template <int TW, int NC>
+struct Atoms {
+    float3 ref[NC];
+    float q[NC];
+    uint32_t mlo[NC], mhi[NC];
+    uint32_t mapoff[NC];      // map index * g3
+    bool valid[NC], ign[NC], lig[NC];
+
+    DEV void load(const LigandDev* L, const GridDev& G) {
+        const char* gb = reinterpret_cast<const char*>(L);
+        const float4* ar = reinterpret_cast<const float4*>(gb + L->off_atom_ref);
+        const int4* ai = reinterpret_cast<const int4*>(gb + L->off_atom_info);
+        const uint2* am = reinterpret_cast<const uint2*>(gb + L->off_atom_mask);
+        const int tl = Team<TW>::lane();
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            int a = tl + TW * k;
+            valid[k] = a < L->num_atoms;
+            int aa = valid[k] ? a : 0;
+            float4 r = ar[aa];
+            int4 inf = ai[aa];
+            uint2 m = am[aa];
+            ref[k] = make_float3(r.x, r.y, r.z);
+            q[k] = r.w;
+            mlo[k] = valid[k] ? m.x : 0u;
+            mhi[k] = valid[k] ? m.y : 0u;
+            mapoff[k] = valid[k] ? (uint32_t)inf.x * G.g3 : 0u;
+            ign[k] = !valid[k] || inf.z != 0;
+            lig[k] = valid[k] && a < L->true_ligand_atoms;
+        }
+    }
+};
+
+template <int TW, int NC>
+struct EvalCtx {
+    const LigandDev* L;
+    GridDev G;
+    SharedView sv;
+    TeamView tv;
+    Atoms<TW, NC> A;
+};
+
+// ---------------------------------------------------------------------------
+// Pose (calcenergy.cl:201-221): genes 0..5 and the sine/cosine of the
+// orientation angles, which the gene lanes left in the team's LDS block.
+// ---------------------------------------------------------------------------
+struct Pose {
+    float4 q;          // genrot quaternion
+    float3 mv;         // translation (grid units)
+    float phi, theta, rotangle, sin_theta, sin_half, is_theta_gt_pi;
+};
+
+// Every gene lane computes the trigonometry of its own angle gene once:
+// orientation genes into pg, torsion genes into the torsion quaternions
+// (calcenergy.cl:201-221, 248-260).  Followed by wave_sync by the caller.
+template <int TW>
+DEV void gene_trig(const LigandDev* L, const SharedView& sv, const TeamView& tv, const float (&gene)[64 / TW]) {
+    const int tl = Team<TW>::lane();
+    const int NG = L->num_genes;
+#pragma unroll
+    for (int j = 0; j < Team<TW>::kGPL; ++j) {
+        const int g = tl + TW * j;
+        if (j > 0 && NG <= TW) break;
+        const float v = gene[j];
+        if (g < 6) tv.pg[g] = v;
+        if (g >= 3 && g < NG) {
+            float ang = v * kDegToRad;
+            if (g >= 5) ang = ang * 0.5f;
+            float s, c;
+            sincosf(ang, &s, &c);
+            if (g < 6) {
+                tv.pg[6 + 2 * (g - 3)] = s;
+                tv.pg[7 + 2 * (g - 3)] = c;
+            } else {
+                float4 u = sv.tor_unit[g - 6];
+                tv.tq[g - 6] = make_float4(s * u.x, s * u.y, s * u.z, c);
+            }
+        }
+    }
+}
+
+DEV Pose read_pose(const TeamView& tv, bool has_ligand) {
+    Pose P;
+    const float4 p0 = *reinterpret_cast<const float4*>(tv.pg);       // g0 g1 g2 g3
+    const float4 p1 = *reinterpret_cast<const float4*>(tv.pg + 4);   // g4 g5 sin(phi) cos(phi)
+    const float4 p2 = *reinterpret_cast<const float4*>(tv.pg + 8);   // sin/cos theta, sin/cos rot/2
+    P.phi = p0.w * kDegToRad;
+    P.theta = p1.x * kDegToRad;
+    P.rotangle = p1.y * kDegToRad;
+    P.q = make_float4(0.f, 0.f, 0.f, 1.f);
+    P.mv = make_float3(0.f, 0.f, 0.f);
+    P.sin_theta = 0.f; P.sin_half = 0.f; P.is_theta_gt_pi = 1.f;
+    if (has_ligand) {
+        const float sp = p1.z, cp = p1.w, st = p2.x, ct = p2.y, sh = p2.z, ch = p2.w;
+        P.mv = make_float3(p0.x, p0.y, p0.z);
+        P.q = make_float4(sh * st * cp, sh * st * sp, sh * ct, ch);
+        P.sin_theta = st;
+        P.sin_half = sh;
+        P.is_theta_gt_pi = 1.0f - 2.0f * (float)(st < 0.0f);
+    }
+    return P;
+}
+
+// Pose -> coordinates (calcenergy.cl:232-275).  Per atom: the rotatable bonds
+// moving it in ascending index, then the general rotation (the per-atom order
+// gen_rotlist encodes, host/src/calcenergy.cpp:387-488).
+template <int TW, int NC>
+DEV void compute_coords(const SharedView& sv, const TeamView& tv, const Atoms<TW, NC>& A, const Pose& P,
+                        float3 (&c)[NC]) {
+    const int tl = Team<TW>::lane();
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        float3 v = A.ref[k];
+        uint32_t lo = A.mlo[k], hi = A.mhi[k];
+        while (lo | hi) {
+            int t;
+            if (lo) { t = __builtin_ctz(lo); lo &= lo - 1; }
+            else { t = 32 + __builtin_ctz(hi); hi &= hi - 1; }
+            float4 q = tv.tq[t];
+            float4 m = sv.tor_move[t];
+            float3 r = qrot(make_float3(v.x - m.x, v.y - m.y, v.z - m.z), q);
+            v = make_float3(r.x + m.x, r.y + m.y, r.z + m.z);
+        }
+        if (A.lig[k]) {
+            float3 r = qrot(v, P.q);
+            v = make_float3(r.x + P.mv.x, r.y + P.mv.y, r.z + P.mv.z);
+        }
+        c[k] = v;
+        tv.coords[tl + TW * k] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Intermolecular energy (+ map gradient) of one atom, calcenergy.cl:280-379,
+// calcMergedEneGra.cl:222-443.  Split in two so the map loads of all atoms
+// are in flight while the pair loop runs.
+// ---------------------------------------------------------------------------
+template <int MF>
+struct Gather {
+    float4 a0, a1, e0, e1, d0, d1;
+    bool in;
+};
+
+template <int MF>
+DEV void gather_issue(const GridDev& G, float3 c, uint32_t mapoff, Gather<MF>& g) {
+    // out of the grid (calcenergy.cl:293-301); NaN coordinates count as out,
+    // so no map index is ever formed from them
+    g.in = (c.x >= 0.f) && (c.y >= 0.f) && (c.z >= 0.f) && (c.x < (float)(G.gx - 1)) &&
+           (c.y < (float)(G.gy - 1)) && (c.z < (float)(G.gz - 1));
+    uint32_t p = 0;
+    if (g.in) p = (uint32_t)floorf(c.x) + (uint32_t)floorf(c.y) * G.g1 + (uint32_t)floorf(c.z) * G.g2;
+    const uint32_t ne = (uint32_t)(G.nmaps - 2) * G.g3, nd = ne + G.g3;
+    g.a0 = load_pt<MF>(G.data, mapoff + p); g.a1 = load_pt<MF>(G.data, mapoff + p + 1);
+    g.e0 = load_pt<MF>(G.data, ne + p); g.e1 = load_pt<MF>(G.data, ne + p + 1);
+    g.d0 = load_pt<MF>(G.data, nd + p); g.d1 = load_pt<MF>(G.data, nd + p + 1);
+}
+
+template <int MF, bool GRAD>
+DEV float gather_finish(const GridDev& G, float3 c, float q, const Gather<MF>& g, int3& fi) {
+    if (!g.in) {
+        if (GRAD) { fi.x += 16777216; fi.y += 16777216; fi.z += 16777216; }
+        return 16777216.0f;
+    }
+    float xl = floorf(c.x), yl = floorf(c.y), zl = floorf(c.z);
+    float dx = c.x - xl, omdx = 1.0f - dx;
+    float dy = c.y - yl, omdy = 1.0f - dy;
+    float dz = c.z - zl, omdz = 1.0f - dz;
+    float w0 = omdx * omdy * omdz, w1 = omdx * dy * omdz, w2 = omdx * omdy * dz, w3 = omdx * dy * dz;
+    float w4 = dx * omdy * omdz, w5 = dx * dy * omdz, w6 = dx * omdy * dz, w7 = dx * dy * dz;
+    const float4 a0 = g.a0, a1 = g.a1, e0 = g.e0, e1 = g.e1, d0 = g.d0, d1 = g.d1;
+    float et = a0.x * w0 + a0.y * w1 + a0.z * w2 + a0.w * w3 + a1.x * w4 + a1.y * w5 + a1.z * w6 + a1.w * w7;
+    float ee = e0.x * w0 + e0.y * w1 + e0.z * w2 + e0.w * w3 + e1.x * w4 + e1.y * w5 + e1.z * w6 + e1.w * w7;
+    float ed = d0.x * w0 + d0.y * w1 + d0.z * w2 + d0.w * w3 + d1.x * w4 + d1.y * w5 + d1.z * w6 + d1.w * w7;
+    float aq = fabsf(q);
+    float e = et + q * ee + aq * ed;
+    if (GRAD) {
+        const float ig = G.inv_spacing;
+        // cube index k: 0 000, 1 010, 2 001, 3 011, 4 100, 5 110, 6 101, 7 111
+        auto gxf = [&](float4 l, float4 h) {
+            return omdz * (omdy * (h.x - l.x) + dy * (h.y - l.y)) + dz * (omdy * (h.z - l.z) + dy * (h.w - l.w));
+        };
+        auto gyf = [&](float4 l, float4 h) {
+            return omdz * (omdx * (l.y - l.x) + dx * (h.y - h.x)) + dz * (omdx * (l.w - l.z) + dx * (h.w - h.z));
+        };
+        auto gzf = [&](float4 l, float4 h) {
+            return omdy * (omdx * (l.z - l.x) + dx * (h.z - h.x)) + dy * (omdx * (l.w - l.y) + dx * (h.w - h.y));
+        };
+        float qg = q * ig, aqg = fabsf(qg);
+        float gx = gxf(a0, a1) * ig + qg * gxf(e0, e1) + aqg * gxf(d0, d1);
+        float gy = gyf(a0, a1) * ig + qg * gyf(e0, e1) + aqg * gyf(d0, d1);
+        float gz = gzf(a0, a1) * ig + qg * gzf(e0, e1) + aqg * gzf(d0, d1);
+        fi.x += quant(kTermScale * gx);
+        fi.y += quant(kTermScale * gy);
+        fi.z += quant(kTermScale * gz);
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// Intramolecular pair energy and dE/dr (calcenergy.cl:397-536,
+// calcMergedEneGra.cl:455-583).
+// ---------------------------------------------------------------------------
+template <bool GRAD>
+DEV float pair_energy(const LigandDev* L, const PairRec& pr, float ad, float& dedr) {
+    float e = 0.f;
+    dedr = 0.f;
+    if (pr.flags & (1u << 16)) { e += 50.0f * ad; dedr += 50.0f; }   // CG-G0 bond term, G = 50
+    if (ad < 8.0f) {
+        float delta = pr.reqm - ad;
+        float sd = (fabsf(delta) >= L->half_smooth) ? ad + copysignf(L->half_smooth, delta) : pr.reqm;
+        int m = (int)(pr.flags & 0xffu), n = (int)((pr.flags >> 8) & 0xffu);
+        float rm, rmn;
+        if (pr.flags & (1u << 17)) {        // m = 12, n in {6, 10}
+            float r2 = sd * sd, r4 = r2 * r2, r6 = r4 * r2;
+            rm = fast_rcp(r6 * r6);
+            rmn = (n == 6) ? r6 : r2;
+        } else {                            // general exponents (modified pairs)
+            float l2 = __builtin_amdgcn_logf(sd);
+            rm = __builtin_amdgcn_exp2f(-(float)m * l2);
+            rmn = __builtin_amdgcn_exp2f((float)(m - n) * l2);
+        }
+        e += (pr.ac - rmn * pr.bd) * rm;
+        if (GRAD) dedr += ((float)n * pr.bd * rmn - (float)m * pr.ac) * rm * fast_rcp(sd);
+    }
+    if (ad < 20.48f) {
+        float d = ad < L->elec_min_distance ? L->elec_min_distance : ad;
+        float d2 = d * d;
+        float desolv = pr.dsol * (L->coeff_desolv * (12.96f - 0.1063f * d2 * (1.0f - 0.001947f * d2)) *
+                                  fast_rcp(12.96f + d2 * (0.4137f + d2 * (0.00357f + 0.000112f * d2))));
+        float ds = d + 1.26366f;
+        float rds2 = fast_rcp(ds * ds);
+        float diel = 1.10859f * rds2 + 0.010358f;
+        float rd = fast_rcp(d);
+        float es = pr.cq * rd;
+        e += diel * es + desolv;
+        if (GRAD) dedr += -es * rd * diel - es * 2.21718f * rds2 * fast_rcp(ds) - 0.0771605f * d * desolv;
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// One evaluation of one genotype by one team.
+//   GRAD = false: gpu_calc_energy (calcenergy.cl:132-554)
+//   GRAD = true : gpu_calc_energrad (calcMergedEneGra.cl:46-936); grad[j]
+//                 receives d E / d gene (lane + TW j), 0 beyond the genes.
+// All lanes of the wavefront must call it together (uniform control flow).
+// ---------------------------------------------------------------------------
+template <int TW, int NC, int MF, bool GRAD>
+DEV float evaluate(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float (&grad)[64 / TW]) {
+    using TM = Team<TW>;
+    const LigandDev* L = X.L;
+    const GridDev& G = X.G;
+    const TeamView& tv = X.tv;
+    const SharedView& sv = X.sv;
+    const int tl = TM::lane();
+    const int T = L->num_rotbonds, P = L->num_pairs, NG = L->num_genes;
+    const bool has_lig = L->true_ligand_atoms != 0;
+
+    gene_trig<TW>(L, sv, tv, gene);
+    if (GRAD) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) { tv.fx[tl + TW * k] = 0; tv.fy[tl + TW * k] = 0; tv.fz[tl + TW * k] = 0; }
+        for (int t = tl; t < T; t += TW) tv.gi[t] = 0;
+    }
+    wave_sync();
+    const Pose pose = read_pose(tv, has_lig);
+    float3 c[NC];
+    compute_coords<TW, NC>(sv, tv, X.A, pose, c);
+
+    // with one atom per lane the map loads go out before the pair loop and
+    // land while it runs; with more, each chunk is gathered after it (the
+    // loads of all chunks would not fit the register budget)
+    Gather<MF> gth0;
+    if constexpr (NC == 1) gather_issue<MF>(G, c[0], X.A.mapoff[0], gth0);
+    wave_sync();
+
+    const float spacing = G.spacing;
+    float e = 0.f;
+    for (int p = tl; p < P; p += TW) {
+        const PairRec pr = sv.pairs[p];
+        const int i = (int)(pr.ij & 0xffffu), j = (int)(pr.ij >> 16);
+        const float4 ci = tv.coords[i], cj = tv.coords[j];
+        const float sx = ci.x - cj.x, sy = ci.y - cj.y, sz = ci.z - cj.z;
+        const float dist = fast_sqrt(sx * sx + sy * sy + sz * sz);
+        float dedr;
+        e += pair_energy<GRAD>(L, pr, dist * spacing, dedr);
+        if (GRAD) {
+            const float gdd = -dedr * fast_rcp(dist);
+            const int ix = quant(kTermScale * sx * gdd), iy = quant(kTermScale * sy * gdd),
+                      iz = quant(kTermScale * sz * gdd);
+            atomicSub(&tv.fx[i], ix); atomicSub(&tv.fy[i], iy); atomicSub(&tv.fz[i], iz);
+            atomicAdd(&tv.fx[j], ix); atomicAdd(&tv.fy[j], iy); atomicAdd(&tv.fz[j], iz);
+        }
+    }
+    int3 fi[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        fi[k] = make_int3(0, 0, 0);
+        if constexpr (NC == 1) {
+            const float ea = gather_finish<MF, GRAD>(G, c[k], X.A.q[k], gth0, fi[k]);
+            if (!X.A.ign[k]) e += ea;
+        } else {
+            Gather<MF> gk;
+            gather_issue<MF>(G, c[k], X.A.mapoff[k], gk);
+            const float ea = gather_finish<MF, GRAD>(G, c[k], X.A.q[k], gk, fi[k]);
+            if (!X.A.ign[k]) e += ea;
+        }
+    }
+    const float energy = TM::sum(e);
+    if (!GRAD) return energy;
+
+    wave_sync();
+    // total per-atom forces (kcal/mol/A), int fixed point -> float
+    float3 F[NC];
+    float sfx = 0.f, sfy = 0.f, sfz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const int a = tl + TW * k;
+        F[k] = make_float3(0.f, 0.f, 0.f);
+        if (X.A.valid[k]) {
+            int fx = tv.fx[a], fy = tv.fy[a], fz = tv.fz[a];
+            if (!X.A.ign[k]) { fx += fi[k].x; fy += fi[k].y; fz += fi[k].z; }
+            F[k] = make_float3(kOneOverTermScale * (float)fx, kOneOverTermScale * (float)fy,
+                               kOneOverTermScale * (float)fz);
+            sfx += F[k].x; sfy += F[k].y; sfz += F[k].z;
+            if (X.A.lig[k]) {   // torque about the ligand centre, calcMergedEneGra.cl:712-745
+                const float rx = (c[k].x - pose.mv.x) * spacing, ry = (c[k].y - pose.mv.y) * spacing,
+                            rz = (c[k].z - pose.mv.z) * spacing;
+                tx += ry * F[k].z - rz * F[k].y;
+                ty += rz * F[k].x - rx * F[k].z;
+                tz += rx * F[k].y - ry * F[k].x;
+            }
+        }
+    }
+    sfx = TM::sum(sfx); sfy = TM::sum(sfy); sfz = TM::sum(sfz);
+    tx = TM::sum(tx); ty = TM::sum(ty); tz = TM::sum(tz);
+    int ig = 0;
+    if (has_lig && tl < 6) {
+        // translation genes, calcMergedEneGra.cl:679-697
+        const int it0 = quant(kTermScale * sfx * spacing), it1 = quant(kTermScale * sfy * spacing),
+                  it2 = quant(kTermScale * sfz * spacing);
+        // rotation genes, calcMergedEneGra.cl:756-861
+        const float tl2 = tx * tx + ty * ty + tz * tz;
+        const float tlen = fast_sqrt(tl2);
+        const bool small = tlen < 1e-3f;
+        const float oscale = small ? 1.0f : tlen * (1.0f / 1e-3f);
+        const float sc = small ? 0.5f - tlen * tlen * (1.0f / 48.0f) : 0.0005f * fast_rcp(tlen);
+        const float4 qt = make_float4(tx * sc, ty * sc, tz * sc, small ? 1.0f - tlen * tlen * 0.125f : 0.99999988f);
+        const float4 tq = qmul(qt, pose.q);
+        const float cur_phi = fmod_pi2(kTwoPi + pose.phi);
+        const float cur_theta = fmod_pi2(kTwoPi + pose.theta);
+        const float cur_rot = fmod_pi2(kTwoPi + pose.rotangle);
+        const float tgt_rot = 2.0f * fast_acos(tq.w);
+        const float inv_sin = fast_rsq(1.0f - tq.w * tq.w);
+        const float tgt_theta = kTwoPi + pose.is_theta_gt_pi * fast_acos(tq.z * inv_sin);
+        const float tgt_phi = fmod_pi2(atan2f(pose.is_theta_gt_pi * tq.y, pose.is_theta_gt_pi * tq.x) + kTwoPi);
+        const float gphi = oscale * (fmod_pi2(tgt_phi - cur_phi + kPi) - kPi);
+        const float gtheta = oscale * (fmod_pi2(tgt_theta - cur_theta + kPi) - kPi);
+        const float grot = oscale * (fmod_pi2(tgt_rot - cur_rot + kPi) - kPi);
+        const float corr = 4.0f * pose.sin_half * pose.sin_half;
+        const int ir0 = quant(kTermScale * gphi * pose.sin_theta * pose.sin_theta * corr * kDegToRad);
+        const int ir1 = quant(kTermScale * gtheta * corr * kDegToRad);
+        const int ir2 = quant(kTermScale * grot * kDegToRad);
+        ig = tl == 0 ? it0 : tl == 1 ? it1 : tl == 2 ? it2 : tl == 3 ? ir0 : tl == 4 ? ir1 : ir2;
+    }
+    // torsion genes, calcMergedEneGra.cl:866-915
+    for (int t = tl; t < T; t += TW) {
+        const int2 ab = sv.tor_atoms[t];
+        const float4 r0 = tv.coords[ab.x], r1 = tv.coords[ab.y];
+        const float ax = r1.x - r0.x, ay = r1.y - r0.y, az = r1.z - r0.z;
+        const float inv = fast_rsq(ax * ax + ay * ay + az * az);
+        tv.tax[t] = make_float4(ax * inv, ay * inv, az * inv, 0.f);
+        tv.tref[t] = r0;
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        uint32_t lo = X.A.mlo[k], hi = X.A.mhi[k];
+        while (lo | hi) {
+            int t;
+            if (lo) { t = __builtin_ctz(lo); lo &= lo - 1; }
+            else { t = 32 + __builtin_ctz(hi); hi &= hi - 1; }
+            const float4 ax = tv.tax[t], r0 = tv.tref[t];
+            const float rx = c[k].x - r0.x, ry = c[k].y - r0.y, rz = c[k].z - r0.z;
+            const float qx = ry * F[k].z - rz * F[k].y, qy = rz * F[k].x - rx * F[k].z,
+                        qz = rx * F[k].y - ry * F[k].x;
+            const float toa = (ax.x * qx + ax.y * qy + ax.z * qz) * spacing;
+            atomicAdd(&tv.gi[t], quant(kTermScale * toa * kDegToRad));
+        }
+    }
+    wave_sync();
+    const float conv = spacing * spacing * 1.0f / (kDegToRad * kDegToRad);   // CONVERT_INTO_ANGSTROM_RADIAN
+#pragma unroll
+    for (int j = 0; j < TM::kGPL; ++j) {
+        const int g = tl + TW * j;
+        int iv = 0;
+        if (g < 6) iv = ig;
+        else if (g < NG) iv = tv.gi[g - 6];
+        float gv = kOneOverTermScale * (float)iv;
+        if (g >= 3) gv *= conv;
+        grad[j] = g < NG ? gv : 0.f;
+    }
+    return energy;
+}
+
+// ---------------------------------------------------------------------------
+// Float gradient with the exact Mehler-Solmajer / exponential desolvation
+// derivatives, used once per FIRE search (calcgradient.cl:109-1003).  Forces
+// are float; intra forces use LDS float atomics of one team.
+// ---------------------------------------------------------------------------
+template <int TW, int NC, int MF>
+DEV void float_gradient(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float (&grad)[64 / TW]) {
+    using TM = Team<TW>;
+    const LigandDev* L = X.L;
+    const GridDev& G = X.G;
+    const TeamView& tv = X.tv;
+    const SharedView& sv = X.sv;
+    const int tl = TM::lane();
+    const int T = L->num_rotbonds, P = L->num_pairs, NG = L->num_genes;
+    const bool has_lig = L->true_ligand_atoms != 0;
+    float* ffx = reinterpret_cast<float*>(tv.fx);
+    float* ffy = reinterpret_cast<float*>(tv.fy);
+    float* ffz = reinterpret_cast<float*>(tv.fz);
+    float* tacc = reinterpret_cast<float*>(tv.gi);
+
+    gene_trig<TW>(L, sv, tv, gene);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) { ffx[tl + TW * k] = 0.f; ffy[tl + TW * k] = 0.f; ffz[tl + TW * k] = 0.f; }
+    for (int t = tl; t < T; t += TW) tacc[t] = 0.f;
+    wave_sync();
+    const Pose pose = read_pose(tv, has_lig);
+    float3 c[NC];
+    compute_coords<TW, NC>(sv, tv, X.A, pose, c);
+    wave_sync();
+    float3 gin[NC];
+    const uint32_t g3 = G.g3, ne = (uint32_t)(G.nmaps - 2) * g3, nd = ne + g3;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        gin[k] = make_float3(0.f, 0.f, 0.f);
+        if (X.A.ign[k]) continue;
+        const float x = c[k].x, y = c[k].y, z = c[k].z;
+        if (!((x >= 0.f) && (y >= 0.f) && (z >= 0.f) && (x < (float)(G.gx - 1)) && (y < (float)(G.gy - 1)) &&
+              (z < (float)(G.gz - 1)))) {
+            gin[k] = make_float3(16777216.0f, 16777216.0f, 16777216.0f);
+            continue;
+        }
+        const uint32_t xl = (uint32_t)floorf(x), yl = (uint32_t)floorf(y), zl = (uint32_t)floorf(z);
+        const uint32_t xh = (uint32_t)ceilf(x), yh = (uint32_t)ceilf(y), zh = (uint32_t)ceilf(z);
+        const float dx = x - (float)xl, dy = y - (float)yl, dz = z - (float)zl;
+        const uint32_t o[8] = {xl + yl * G.g1 + zl * G.g2, xl + yh * G.g1 + zl * G.g2, xl + yl * G.g1 + zh * G.g2,
+                               xl + yh * G.g1 + zh * G.g2, xh + yl * G.g1 + zl * G.g2, xh + yh * G.g1 + zl * G.g2,
+                               xh + yl * G.g1 + zh * G.g2, xh + yh * G.g1 + zh * G.g2};
+        const uint32_t mb[3] = {X.A.mapoff[k], ne, nd};
+        const float sc[3] = {1.0f, X.A.q[k], fabsf(X.A.q[k])};
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            float cb[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cb[i] = load_pt<MF>(G.data, mb[m] + o[i]).x;
+            const float vx0 = (1.0f - dy) * (cb[4] - cb[0]) + dy * (cb[5] - cb[1]);
+            const float vx1 = (1.0f - dy) * (cb[6] - cb[2]) + dy * (cb[7] - cb[3]);
+            const float vy0 = (1.0f - dx) * (cb[1] - cb[0]) + dx * (cb[5] - cb[4]);
+            const float vy1 = (1.0f - dx) * (cb[3] - cb[2]) + dx * (cb[7] - cb[6]);
+            const float vz0 = (1.0f - dx) * (cb[2] - cb[0]) + dx * (cb[6] - cb[4]);
+            const float vz1 = (1.0f - dx) * (cb[3] - cb[1]) + dx * (cb[7] - cb[5]);
+            gin[k].x += sc[m] * ((1.0f - dz) * vx0 + dz * vx1);
+            gin[k].y += sc[m] * ((1.0f - dz) * vy0 + dz * vy1);
+            gin[k].z += sc[m] * ((1.0f - dy) * vz0 + dy * vz1);
+        }
+    }
+    const float spacing = G.spacing;
+    const float half_smooth = L->half_smooth;
+    const float DA = -8.5525f, DB = 78.4f - (-8.5525f), DK = 7.7839f;
+    const float DBH = DB * 0.003627f, DBHK = DBH * DK;
+    for (int p = tl; p < P; p += TW) {
+        const PairRec pr = sv.pairs[p];
+        const int i = (int)(pr.ij & 0xffffu), j = (int)(pr.ij >> 16);
+        const float4 ci = tv.coords[i], cj = tv.coords[j];
+        const float sx = ci.x - cj.x, sy = ci.y - cj.y, sz = ci.z - cj.z;
+        const float dist = sqrtf(sx * sx + sy * sy + sz * sz);
+        float d = dist * spacing;
+        float sd;
+        if (d <= pr.reqm - half_smooth) sd = d + half_smooth;
+        else if (d < pr.reqm + half_smooth) sd = pr.reqm;
+        else sd = d - half_smooth;
+        const int m = (int)(pr.flags & 0xffu), n = (int)((pr.flags >> 8) & 0xffu);
+        float pg = 0.f;
+        if (d < 8.0f) {
+            pg += (-(float)m * pr.ac) / powf(sd, (float)(m + 1));
+            pg += ((float)n * pr.bd) / powf(sd, (float)(n + 1));
+        }
+        if (d < 20.48f) {
+            if (d < L->elec_min_distance) d = L->elec_min_distance;
+            const float ex = __expf(DBH * d);
+            const float upper = DA * (ex + DK) * (ex + DK) + DB * ex * (DBHK * d + ex + DK);
+            const float lw = DA * (ex + DK) + DB * ex;
+            const float lower = d * d * lw * lw;
+            pg += -pr.cq * (upper / lower);
+            pg += pr.dsol * L->coeff_desolv * -0.077160f * d * __expf(-0.038580f * d * d);
+        }
+        if (pr.flags & (1u << 16)) pg += 50.0f;
+        const float fx = pg * (-sx / dist), fy = pg * (-sy / dist), fz = pg * (-sz / dist);
+        atomicAdd(&ffx[i], -fx); atomicAdd(&ffy[i], -fy); atomicAdd(&ffz[i], -fz);
+        atomicAdd(&ffx[j], fx); atomicAdd(&ffy[j], fy); atomicAdd(&ffz[j], fz);
+    }
+    wave_sync();
+    float3 F[NC];
+    float sfx = 0.f, sfy = 0.f, sfz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
+    const float g0 = tv.pg[0], g1 = tv.pg[1], g2 = tv.pg[2];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const int a = tl + TW * k;
+        F[k] = make_float3(0.f, 0.f, 0.f);
+        if (X.A.valid[k]) {
+            F[k] = make_float3(gin[k].x / spacing + ffx[a], gin[k].y / spacing + ffy[a], gin[k].z / spacing + ffz[a]);
+            if (X.A.lig[k]) {
+                sfx += F[k].x; sfy += F[k].y; sfz += F[k].z;
+                const float rx = (c[k].x - g0) * spacing, ry = (c[k].y - g1) * spacing, rz = (c[k].z - g2) * spacing;
+                tx += ry * F[k].z - rz * F[k].y;
+                ty += rz * F[k].x - rx * F[k].z;
+                tz += rx * F[k].y - ry * F[k].x;
+            }
+        }
+    }
+    sfx = TM::sum(sfx); sfy = TM::sum(sfy); sfz = TM::sum(sfz);
+    tx = TM::sum(tx); ty = TM::sum(ty); tz = TM::sum(tz);
+    float gval = 0.f;
+    if (tl < 3) gval = (tl == 0 ? sfx : tl == 1 ? sfy : sfz) * spacing;
+    if (has_lig && tl >= 3 && tl < 6) {   // calcgradient.cl:714-873 (fmod)
+        const float tlen = sqrtf(tx * tx + ty * ty + tz * tz);
+        const bool small = tlen < 1e-3f;
+        const float oscale = small ? 1.0f : tlen * (1.0f / 1e-3f);
+        const float sc = small ? 0.5f - tlen * tlen / 48.0f : 0.0005f / tlen;
+        const float4 qt = make_float4(tx * sc, ty * sc, tz * sc, small ? 1.0f - tlen * tlen * 0.125f : 0.99999988f);
+        const float4 tq = qmul(qt, pose.q);
+        const float cur_phi = fmod_pi2(kTwoPi + pose.phi);
+        const float cur_theta = fmod_pi2(kTwoPi + pose.theta);
+        const float cur_rot = fmod_pi2(kTwoPi + pose.rotangle);
+        const float tgt_rot = 2.0f * fast_acos(tq.w);
+        const float inv_sin = rsqrtf(1.0f - tq.w * tq.w);
+        const float tgt_theta = kTwoPi + pose.is_theta_gt_pi * fast_acos(tq.z * inv_sin);
+        const float tgt_phi = fmod_pi2(atan2f(pose.is_theta_gt_pi * tq.y, pose.is_theta_gt_pi * tq.x) + kTwoPi);
+        const float corr = 4.0f * pose.sin_half * pose.sin_half;
+        if (tl == 3) gval = oscale * (fmodf(tgt_phi - cur_phi + kPi, kTwoPi) - kPi) * pose.sin_theta * pose.sin_theta * corr * kDegToRad;
+        if (tl == 4) gval = oscale * (fmodf(tgt_theta - cur_theta + kPi, kTwoPi) - kPi) * corr * kDegToRad;
+        if (tl == 5) gval = oscale * (fmodf(tgt_rot - cur_rot + kPi, kTwoPi) - kPi) * kDegToRad;
+    }
+    for (int t = tl; t < T; t += TW) {
+        const int2 ab = sv.tor_atoms[t];
+        const float4 r0 = tv.coords[ab.x], r1 = tv.coords[ab.y];
+        const float ax = r1.x - r0.x, ay = r1.y - r0.y, az = r1.z - r0.z;
+        const float inv = rsqrtf(ax * ax + ay * ay + az * az);
+        tv.tax[t] = make_float4(ax * inv, ay * inv, az * inv, 0.f);
+        tv.tref[t] = r0;
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        uint32_t lo = X.A.mlo[k], hi = X.A.mhi[k];
+        while (lo | hi) {
+            int t;
+            if (lo) { t = __builtin_ctz(lo); lo &= lo - 1; }
+            else { t = 32 + __builtin_ctz(hi); hi &= hi - 1; }
+            const float4 ax = tv.tax[t], r0 = tv.tref[t];
+            const float rx = (c[k].x - r0.x) * spacing, ry = (c[k].y - r0.y) * spacing, rz = (c[k].z - r0.z) * spacing;
+            const float qx = ry * F[k].z - rz * F[k].y, qy = rz * F[k].x - rx * F[k].z, qz = rx * F[k].y - ry * F[k].x;
+            atomicAdd(&tacc[t], (ax.x * qx + ax.y * qy + ax.z * qz) * kDegToRad);
+        }
+    }
+    wave_sync();
+    const float conv = spacing * spacing * 1.0f / (kDegToRad * kDegToRad);
+#pragma unroll
+    for (int j = 0; j < TM::kGPL; ++j) {
+        const int g = tl + TW * j;
+        float gv = 0.f;
+        if (g < 6) gv = gval;
+        else if (g < NG) gv = tacc[g - 6];
+        if (g > 2) gv *= conv;
+        grad[j] = g < NG ? gv : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel arguments and common prologue
+// ---------------------------------------------------------------------------
+struct Launch {
+    const LigandDev* L;
+    GridDev G;
+    SearchDev S;
+    JobBuffers B;
+    const float* in_geno;
+    float* out_e;
+    float* out_g;
+    int n;
+    int mode;
+    float lig_scale;     // SW: 1/sqrt(atoms)
+    float gene_scale;    // SW: 1/sqrt(genes)
+};
+
+#define KERNEL_PROLOGUE(TW, NC, PL)                                            \
+    extern __shared__ __align__(16) char lds[];                                \
+    using TM = Team<TW>;                                                       \
+    constexpr int GPL = TM::kGPL;                                              \
+    EvalCtx<TW, NC> X;                                                         \
+    X.L = a.L;                                                                 \
+    X.G = a.G;                                                                 \
+    stage<TW, NC, PL>(a.L, lds, X.sv, X.tv);                                   \
+    X.A.load(a.L, a.G);                                                        \
+    const int tl = TM::lane();                                                 \
+    (void)tl;                                                                  \
+    const int team_idx = blockIdx.x * TM::kPerGroup + (int)(threadIdx.x / TW); \
+    (void)team_idx;
+
+// Entity choice of every LS kernel (e.g. kernel3.cl:104-117): team lane 0
+// draws from its own stream (state *st0) when le == 0.
+template <int TW>
+DEV int ls_entity(const SearchDev& S, int le, uint32_t* st0) {
+    int ent = le;
+    if (le == 0) {
+        float r = 0.f;
+        if (Team<TW>::lane() == 0) { uint32_t s = *st0; r = randf(s); *st0 = s; }
+        r = Team<TW>::bcast(r, 0);
+        if (100.0f * r > S.lsearch_rate) ent = S.num_ls;
+    }
+    // the reference reads entity num_ls even when num_ls == pop_size (the next
+    // run's first entity, or past the buffer for the last run); clamp instead
+    return ent < S.pop_size ? ent : S.pop_size - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+struct Shape {
+    int tw, nc;
+    bool pl;
+    size_t lds;
+};
+
+constexpr size_t kLdsBudget = 64 * 1024;
+
+// Chooses the team width and atom chunks for a ligand: 32-lane teams (two
+// individuals per wavefront) up to 128 atoms, 64-lane teams beyond.
+inline Shape choose_shape(const LigandDev& L, int team_width_hint) {
+    Shape s{};
+    const int N = L.num_atoms;
+    int tw = team_width_hint == 64 ? 64 : (team_width_hint == 32 ? 32 : (N <= 128 ? 32 : 64));
+    if (tw == 32 && N > 128) tw = 64;
+    s.tw = tw;
+    const int per = (N + tw - 1) / tw;
+    s.nc = per <= 1 ? 1 : (per <= 2 ? 2 : 4);
+    s.pl = lds_bytes_for(s.tw, s.nc, L.num_rotbonds, L.num_pairs, true) <= kLdsBudget;
+    s.lds = lds_bytes_for(s.tw, s.nc, L.num_rotbonds, L.num_pairs, s.pl);
+    return s;
+}
+
+// Dispatches a kernel template over (TW, NC, MF, PL).  KT must be a
+// template<int TW, int NC, int MF, bool PL> __global__ function.
+#define ADGPU_DISPATCH(KT, shape, mf, body)                                                     \
+    do {                                                                                        \
+        auto go_ = [&](auto tw_, auto nc_, auto mf_, auto pl_) {                                 \
+            constexpr int TW_ = decltype(tw_)::value, NC_ = decltype(nc_)::value;               \
+            constexpr int MF_ = decltype(mf_)::value;                                           \
+            constexpr bool PL_ = decltype(pl_)::value;                                          \
+            auto kfn = KT<TW_, NC_, MF_, PL_>;                                                  \
+            body;                                                                               \
+        };                                                                                      \
+        auto mf_sel = [&](auto tw_, auto nc_, auto pl_) {                                       \
+            if ((mf) == ADGPU_MAP_F16) go_(tw_, nc_, std::integral_constant<int, ADGPU_MAP_F16>{}, pl_); \
+            else if ((mf) == ADGPU_MAP_BF16) go_(tw_, nc_, std::integral_constant<int, ADGPU_MAP_BF16>{}, pl_); \
+            else go_(tw_, nc_, std::integral_constant<int, ADGPU_MAP_F32>{}, pl_);             \
+        };                                                                                      \
+        auto pl_sel = [&](auto tw_, auto nc_) {                                                 \
+            if ((shape).pl) mf_sel(tw_, nc_, std::true_type{});                                 \
+            else mf_sel(tw_, nc_, std::false_type{});                                           \
+        };                                                                                      \
+        using I32 = std::integral_constant<int, 32>;                                            \
+        using I64 = std::integral_constant<int, 64>;                                            \
+        using N1 = std::integral_constant<int, 1>;                                              \
+        using N2 = std::integral_constant<int, 2>;                                              \
+        using N4 = std::integral_constant<int, 4>;                                              \
+        if ((shape).tw == 32) {                                                                 \
+            if ((shape).nc == 1) pl_sel(I32{}, N1{});                                           \
+            else if ((shape).nc == 2) pl_sel(I32{}, N2{});                                      \
+            else pl_sel(I32{}, N4{});                                                           \
+        } else {                                                                                \
+            if ((shape).nc == 1) pl_sel(I64{}, N1{});                                           \
+            else if ((shape).nc == 2) pl_sel(I64{}, N2{});                                      \
+            else pl_sel(I64{}, N4{});                                                           \
+        }                                                                                       \
+    } while (0)
+
+}  // namespace adgpu

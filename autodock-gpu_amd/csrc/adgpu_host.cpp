@@ -7,8 +7,10 @@
 // population pointer flip, stopped by check_progress (:908-942).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -44,6 +46,7 @@ struct adgpu_context {
     int next_id = 1;
     std::string err;
     std::mutex mu;
+    int team_hint = 0;    // 0 = automatic team width, else 32 or 64 (ADGPU_TEAM_WIDTH)
     // scratch for adgpu_eval_poses
     float* d_geno = nullptr;
     float* d_ene = nullptr;
@@ -65,6 +68,16 @@ struct adgpu_job {
 };
 
 namespace {
+
+LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms) {
+    LaunchArgs la{};
+    la.stream = c->stream;
+    la.lig_dev = reinterpret_cast<const LigandDev*>(ls.dev);
+    la.lig_host = &ls.host;
+    la.grid = ms.dev;
+    la.team_hint = c->team_hint;
+    return la;
+}
 
 int fail(adgpu_context* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -116,7 +129,7 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
     hd.num_rotbonds = T;
     hd.num_genes = T + 6;
     hd.num_pairs = P;
-    hd.nchunk = N <= 64 ? 1 : (N <= 128 ? 2 : 4);
+    hd.nchunk = (N + 31) / 32;
     hd.coeff_elec = ep->coeff_elec;
     hd.coeff_desolv = ep->coeff_desolv;
     hd.qasp = ep->qasp;
@@ -155,8 +168,22 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
         if (a0 < 0 || a0 >= N || a1 < 0 || a1 >= N) return fail(c, ADGPU_ERR_ARG, "rotbond atom out of range");
         ta[t] = make_int2(a0, a1);
     }
+    // Pair order on the device: by index distance j - i, then i.  Consecutive
+    // lanes then handle pairs with distinct first and distinct second atoms, so
+    // the per-atom LDS force accumulations of one instruction rarely collide
+    // (the reference's pair list runs i-major, where up to N-1 lanes would hit
+    // the same atom).  Only the order of the FP32 energy sum changes.
+    std::vector<int> order(P);
+    for (int p = 0; p < P; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        int dx = std::abs(L->pairs[2 * x + 1] - L->pairs[2 * x]), dy = std::abs(L->pairs[2 * y + 1] - L->pairs[2 * y]);
+        if (dx != dy) return dx < dy;
+        return std::min(L->pairs[2 * x], L->pairs[2 * x + 1]) < std::min(L->pairs[2 * y], L->pairs[2 * y + 1]);
+    });
     PairRec* pr = reinterpret_cast<PairRec*>(blob.data() + hd.off_pairs);
-    for (int p = 0; p < P; ++p) {
+    for (int q = 0; q < P; ++q) {
+        const int p = order[q];
+        PairRec& rec = pr[q];
         int i = L->pairs[2 * p], j = L->pairs[2 * p + 1];
         if (i < 0 || i >= N || j < 0 || j >= N) return fail(c, ADGPU_ERR_ARG, "pair atom out of range");
         int t1 = L->atom_types[i], t2 = L->atom_types[j];
@@ -165,16 +192,16 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
         uint32_t r1 = L->atype_reqm[t1], r2 = L->atype_reqm[t2];
         bool cgg0 = (r1 == 24 && r2 == 25) || (r1 == 25 && r2 == 24);   // ATYPE_CG_IDX / ATYPE_G0_IDX
         bool fast = (m == 12) && (n == 6 || n == 10);
-        pr[p].ij = (uint32_t)i | ((uint32_t)j << 16);
-        pr[p].flags = m | (n << 8) | ((uint32_t)cgg0 << 16) | ((uint32_t)fast << 17);
-        pr[p].ac = L->vw_ac[idx];
-        pr[p].bd = L->vw_bd[idx];
-        pr[p].reqm = L->reqm_ab[idx];
+        rec.ij = (uint32_t)i | ((uint32_t)j << 16);
+        rec.flags = m | (n << 8) | ((uint32_t)cgg0 << 16) | ((uint32_t)fast << 17);
+        rec.ac = L->vw_ac[idx];
+        rec.bd = L->vw_bd[idx];
+        rec.reqm = L->reqm_ab[idx];
         float q1 = L->charges[i], q2 = L->charges[j];
-        pr[p].cq = ep->coeff_elec * q1 * q2;                       // calcenergy.cl:527 order
-        pr[p].dsol = (L->ds_s[t1] + ep->qasp * std::fabs(q1)) * L->ds_v[t2] +
-                     (L->ds_s[t2] + ep->qasp * std::fabs(q2)) * L->ds_v[t1];   // calcenergy.cl:488-491
-        pr[p].pad = 0.f;
+        rec.cq = ep->coeff_elec * q1 * q2;                       // calcenergy.cl:527 order
+        rec.dsol = (L->ds_s[t1] + ep->qasp * std::fabs(q1)) * L->ds_v[t2] +
+                   (L->ds_s[t2] + ep->qasp * std::fabs(q2)) * L->ds_v[t1];   // calcenergy.cl:488-491
+        rec.pad = 0.f;
     }
     std::memcpy(blob.data(), &hd, sizeof(hd));
     return ADGPU_OK;
@@ -221,6 +248,14 @@ const char* adgpu_version(void) { return "adgpu-mi355x 0.1 (gfx950)"; }
 
 const char* adgpu_last_error(adgpu_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int adgpu_set_team_width(adgpu_ctx c, int width) {
+    if (!c) return ADGPU_ERR_ARG;
+    if (width != 0 && width != 32 && width != 64) return fail(c, ADGPU_ERR_ARG, "team width must be 0, 32 or 64");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->team_hint = width;
+    return ADGPU_OK;
+}
+
 int adgpu_open(int device, adgpu_ctx* out) {
     if (!out) return ADGPU_ERR_ARG;
     *out = nullptr;
@@ -232,6 +267,10 @@ int adgpu_open(int device, adgpu_ctx* out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess) return ADGPU_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
+    if (const char* tw = std::getenv("ADGPU_TEAM_WIDTH")) {
+        int v = std::atoi(tw);
+        c->team_hint = (v == 32 || v == 64) ? v : 0;
+    }
     *out = c.release();
     return ADGPU_OK;
 }
@@ -365,8 +404,7 @@ int adgpu_eval_poses(adgpu_ctx c, int map_id, int lig_id, int mode, const float*
         c->scratch_n = n;
     }
     HIPCHK(c, hipMemcpyAsync(c->d_geno, genotypes, sizeof(float) * kGenes * (size_t)n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_eval_poses(c->stream, (const LigandDev*)li->second.dev, li->second.host, mi->second.dev, mode,
-                                c->d_geno, n, c->d_ene, c->d_grad));
+    HIPCHK(c, launch_eval_poses(launch_args(c, li->second, mi->second), mode, c->d_geno, n, c->d_ene, c->d_grad));
     if (energies && mode != ADGPU_EVAL_GRADIENT)
         HIPCHK(c, hipMemcpyAsync(energies, c->d_ene, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     if (grads && mode != ADGPU_EVAL_ENERGY)
@@ -403,7 +441,13 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
               hipMalloc(&j->b.evals_new, ne * sizeof(int32_t)) == hipSuccess &&
               hipMalloc(&j->b.evals_runs, sp->num_runs * sizeof(int32_t)) == hipSuccess &&
               hipMalloc(&j->b.prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&j->b.ls_counter, 64 * sizeof(int32_t)) == hipSuccess &&
               hipHostMalloc(&j->h_evals_runs, sp->num_runs * sizeof(int32_t)) == hipSuccess;
+    if (ok && sp->ls_method == ADGPU_LS_FIRE) {
+        const size_t nls = (size_t)sp->num_runs * (j->sd.num_ls > 0 ? j->sd.num_ls : 1);
+        ok = hipMalloc(&j->b.ls_ent, nls * sizeof(int32_t)) == hipSuccess &&
+             hipMalloc(&j->b.ls_grad, nls * kGenes * sizeof(float)) == hipSuccess;
+    }
     if (!ok) {
         adgpu_job_destroy(j.release());
         return fail(c, ADGPU_ERR_NOMEM, "job allocation failed");
@@ -417,6 +461,7 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     HIPCHK(c, hipMemset(j->b.evals_new, 0, ne * sizeof(int32_t)));
     HIPCHK(c, hipMemset(j->b.ene_cur, 0, ne * sizeof(float)));
     HIPCHK(c, hipMemset(j->b.ene_next, 0, ne * sizeof(float)));
+    HIPCHK(c, hipMemset(j->b.ls_counter, 0, 64 * sizeof(int32_t)));
     *out = j.release();
     return ADGPU_OK;
 }
@@ -427,6 +472,7 @@ int adgpu_job_destroy(adgpu_job_t j) {
     (void)hipFree(j->b.pop_cur); (void)hipFree(j->b.pop_next);
     (void)hipFree(j->b.ene_cur); (void)hipFree(j->b.ene_next);
     (void)hipFree(j->b.evals_new); (void)hipFree(j->b.evals_runs); (void)hipFree(j->b.prng);
+    (void)hipFree(j->b.ls_counter); (void)hipFree(j->b.ls_ent); (void)hipFree(j->b.ls_grad);
     if (j->h_evals_runs) (void)hipHostFree(j->h_evals_runs);
     for (auto& e : j->ev) if (e) (void)hipEventDestroy(e);
     delete j;
@@ -450,7 +496,7 @@ int adgpu_job_initpop(adgpu_job_t j) {
     int rc = job_lookup(j, &ls, &ms);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, launch_initpop(c->stream, (const LigandDev*)ls->dev, ls->host, ms->dev, j->sd, j->b));
+    HIPCHK(c, launch_initpop(launch_args(c, *ls, *ms), j->sd, j->b));
     HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ADGPU_OK;
@@ -459,7 +505,7 @@ int adgpu_job_initpop(adgpu_job_t j) {
 static int run_gen(adgpu_job_t j, const LigandSet* ls, const MapSet* ms) {
     adgpu_context* c = j->ctx;
     HIPCHK(c, hipEventRecord(j->ev[0], c->stream));
-    HIPCHK(c, launch_gen(c->stream, (const LigandDev*)ls->dev, ls->host, ms->dev, j->sd, j->b));
+    HIPCHK(c, launch_gen(launch_args(c, *ls, *ms), j->sd, j->b));
     HIPCHK(c, hipEventRecord(j->ev[1], c->stream));
     return ADGPU_OK;
 }
@@ -478,7 +524,7 @@ static int run_ls(adgpu_job_t j, const LigandSet* ls, const MapSet* ms, int64_t 
     int m = ls_method_for(j, generation);
     if (m == ADGPU_LS_NONE) return ADGPU_OK;
     HIPCHK(c, hipEventRecord(j->ev[2], c->stream));
-    HIPCHK(c, launch_ls(c->stream, (const LigandDev*)ls->dev, ls->host, ms->dev, j->sd, j->b, m));
+    HIPCHK(c, launch_ls(launch_args(c, *ls, *ms), j->sd, j->b, m));
     HIPCHK(c, hipEventRecord(j->ev[3], c->stream));
     return ADGPU_OK;
 }

@@ -9,8 +9,8 @@
 
 namespace adgpu {
 
-constexpr int kWave = 64;            // CDNA wavefront width; one wave per individual
-constexpr int kWavesPerGroup = 4;    // individuals per workgroup (they share staged ligand tables)
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int kGroupThreads = 256;   // threads per workgroup (its teams share the staged ligand tables)
 constexpr int kGenes = ADGPU_GENOTYPE_STRIDE;
 
 // One intramolecular pair, precomputed on the host from the reference's
@@ -69,19 +69,34 @@ struct JobBuffers {
     int32_t* evals_new;
     int32_t* evals_runs;
     uint32_t* prng;
+    int32_t* ls_counter;   // work queue of the persistent local-search kernels
+    int32_t* ls_ent;       // FIRE: entity chosen per LS slot [runs*num_ls]
+    float* ls_grad;        // FIRE: initial gradient per LS slot [runs*num_ls*64]
 };
 
-// Launchers (adgpu_kernels.hip).
-hipError_t launch_eval_poses(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
-                             const GridDev& grid, int mode, const float* genotypes, int n,
-                             float* energies, float* grads);
-hipError_t launch_initpop(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
-                          const GridDev& grid, const SearchDev& sp, const JobBuffers& b);
-hipError_t launch_gen(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
-                      const GridDev& grid, const SearchDev& sp, const JobBuffers& b);
-hipError_t launch_ls(hipStream_t s, const LigandDev* lig_dev, const LigandDev& lig_host,
-                     const GridDev& grid, const SearchDev& sp, const JobBuffers& b, int method);
+// What every launcher needs about the ligand and the maps.
+struct LaunchArgs {
+    hipStream_t stream;
+    const LigandDev* lig_dev;    // device copy of the ligand block
+    const LigandDev* lig_host;   // host copy of its header
+    GridDev grid;
+    int team_hint;               // 0 = automatic, 32 or 64 = team width
+};
+
+// Launchers (k_*.hip).
+hipError_t launch_eval_poses(const LaunchArgs& la, int mode, const float* genotypes, int n, float* energies,
+                             float* grads);
+hipError_t launch_initpop(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
+hipError_t launch_gen(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
 hipError_t launch_sum_evals(hipStream_t s, const SearchDev& sp, const JobBuffers& b);
-size_t shared_bytes(const LigandDev& lig_host, bool pairs_in_lds);
+// One local-search launch (resets the work queue first; FIRE also runs its
+// initial-gradient pass).
+hipError_t launch_ls(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b, int method);
+hipError_t launch_ls_sw(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
+hipError_t launch_ls_ad(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
+hipError_t launch_ls_adam(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
+hipError_t launch_ls_fire(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
+// LDS bytes and team width the launchers use for a ligand.
+size_t lds_bytes(const LigandDev& lig_host, int team_hint, int* team_width);
 
 }  // namespace adgpu

@@ -155,6 +155,11 @@ int  adgpu_close(adgpu_ctx ctx);
 const char* adgpu_last_error(adgpu_ctx ctx);
 /* Version string; also usable without a GPU. */
 const char* adgpu_version(void);
+/* Lanes that evaluate one individual: 0 = chosen per ligand (32 up to 128
+ * atoms, 64 above), 32 (two individuals per wavefront) or 64.  Plays the role
+ * of the reference's build-time NUM_OF_THREADS_PER_BLOCK (common/defines.h:29-49)
+ * as a run-time choice; results agree within the FP32 tolerance either way. */
+int  adgpu_set_team_width(adgpu_ctx ctx, int width);
 
 /* --- resident inputs ---------------------------------------------------------
  * Replace the per-job uploads in docking_with_gpu

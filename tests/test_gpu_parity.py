@@ -181,3 +181,30 @@ def test_million_scale_batch_properties(gpu_ctx):
     e3, g3 = gpu_ctx.eval_poses(mid, lid, poses[perm], adgpu.EVAL_ENERGRAD)
     assert np.array_equal(e3, e1[perm]) and np.array_equal(g3, g1[perm])
     helpers.check_energies(case.orc, poses[perm[:400]], e1[perm[:400]])
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_team_width_parity(gpu_ctx, width):
+    """Both team widths (two individuals per wavefront, or one) meet the
+    energy bar and the gradient bar against the oracle, for a ligand that
+    needs one atom chunk per lane at width 64 and three at width 32."""
+    case = helpers.synthetic_case(90, 20, 5, 13, npts=64)
+    mid, lid = _upload(gpu_ctx, case)
+    poses = helpers.poses_for(case, 400, seed=505)
+    gpu_ctx.set_team_width(width)
+    try:
+        e, _ = gpu_ctx.eval_poses(mid, lid, poses, adgpu.EVAL_ENERGY)
+        e2, g = gpu_ctx.eval_poses(mid, lid, poses, adgpu.EVAL_ENERGRAD)
+    finally:
+        gpu_ctx.set_team_width(0)
+    helpers.check_energies(case.orc, poses, e)
+    helpers.check_energies(case.orc, poses, e2, e_ref=e)
+    ng = case.ng
+    qv = np.array([helpers.gene_quantum(case, k) for k in range(ng)])
+    well = [0, 1, 2] + list(range(6, ng))
+    viol = 0
+    for i, p in enumerate(poses):
+        _, go = case.orc.energrad32(p, True)
+        d = np.abs(g[i, :ng] - go)
+        viol += int(np.sum(d[well] > (1e-4 * np.abs(go) + 4 * qv)[well]))
+    assert viol <= 0.002 * len(poses) * len(well), viol
