@@ -152,9 +152,15 @@ class Job:
     def generation(self):
         self._c(self.lib.adgpu_job_generation(self.h), "adgpu_job_generation")
 
-    def run(self):
+    def run(self, max_new_generations=None):
+        """Generations until check_progress reaches 100 % (or at most
+        max_new_generations more); returns (generations so far, total evals)."""
         g, t = C.c_int64(), C.c_int64()
-        self._c(self.lib.adgpu_job_run(self.h, C.byref(g), C.byref(t)), "adgpu_job_run")
+        if max_new_generations is None:
+            self._c(self.lib.adgpu_job_run(self.h, C.byref(g), C.byref(t)), "adgpu_job_run")
+        else:
+            self._c(self.lib.adgpu_job_run_generations(self.h, int(max_new_generations), C.byref(g), C.byref(t)),
+                    "adgpu_job_run_generations")
         return g.value, t.value
 
     def download(self):

@@ -123,10 +123,11 @@ DEV float fast_acos(float cosine) {
 
 // float2int_round of the clamped term, calcMergedEneGra.cl:30-35 (the half is
 // moved toward zero before truncation, as the reference does).
+// v - copysign(0.5, v) equals the reference's v + 0.5 * (v < 0 ? 1 : v > 0 ? -1 : 0)
+// for every v, including +-0.
 DEV int quant(float x) {
-    float v = fminf(kMaxTerm, fmaxf(-kMaxTerm, x));
-    int addsub = (int)(v < 0.0f) - (int)(v > 0.0f);
-    return (int)(v + 0.5f * (float)addsub);
+    const float v = fminf(kMaxTerm, fmaxf(-kMaxTerm, x));
+    return (int)(v - copysignf(0.5f, v));
 }
 
 // quaternion_rotate, calcenergy.cl:119-127
@@ -191,14 +192,17 @@ DEV float4 load_pt<ADGPU_MAP_BF16>(const void* base, uint32_t idx) {
 // LDS layout
 // ---------------------------------------------------------------------------
 struct SharedView {           // staged once per workgroup
-    const PairRec* pairs;     // LDS (or global when the pairs do not fit)
+    const PairA* pa;          // LDS (or global when the pairs do not fit)
+    const PairB* pb;
     const float4* tor_unit;
     const float4* tor_move;
     const int2* tor_atoms;
 };
 
 struct TeamView {             // private to one team
-    float4* coords;           // [TW*NC] atom coordinates (grid units)
+    float* cx;                // [TW*NC] atom coordinates (grid units), one
+    float* cy;                //   array per axis: lanes reading different
+    float* cz;                //   atoms hit different banks
     int* fx;                  // [TW*NC] int32 fixed-point pair forces
     int* fy;
     int* fz;
@@ -212,14 +216,13 @@ struct TeamView {             // private to one team
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 __host__ __device__ inline size_t shared_bytes_for(int T, int P, bool pairs_in_lds) {
-    size_t s = pairs_in_lds ? (size_t)P * sizeof(PairRec) : 0;
+    size_t s = pairs_in_lds ? (size_t)P * (sizeof(PairA) + sizeof(PairB)) : 0;
     s += (size_t)T * (sizeof(float4) * 2 + sizeof(int2));
     return align16(s);
 }
 
 __host__ __device__ inline size_t team_bytes_for(int natoms_pad, int T) {
-    return align16((size_t)natoms_pad * 16) + 3 * align16((size_t)natoms_pad * 4) + (size_t)T * 48 +
-           align16((size_t)T * 4) + 48;
+    return 6 * align16((size_t)natoms_pad * 4) + (size_t)T * 48 + align16((size_t)T * 4) + 48;
 }
 
 __host__ __device__ inline size_t lds_bytes_for(int TW, int NC, int T, int P, bool pairs_in_lds) {
@@ -233,16 +236,18 @@ DEV void stage(const LigandDev* L, char* lds, SharedView& sv, TeamView& tv) {
     const char* gbase = reinterpret_cast<const char*>(L);
     const int T = L->num_rotbonds, P = L->num_pairs;
     char* p = lds;
-    const PairRec* gpairs = reinterpret_cast<const PairRec*>(gbase + L->off_pairs);
+    const PairA* gpa = reinterpret_cast<const PairA*>(gbase + L->off_pair_a);
+    const PairB* gpb = reinterpret_cast<const PairB*>(gbase + L->off_pair_b);
     if (PL) {
-        PairRec* lp = reinterpret_cast<PairRec*>(p);
-        const uint4* src = reinterpret_cast<const uint4*>(gpairs);
-        uint4* dst = reinterpret_cast<uint4*>(lp);
-        for (int k = threadIdx.x; k < 2 * P; k += blockDim.x) dst[k] = src[k];
-        sv.pairs = lp;
-        p += (size_t)P * sizeof(PairRec);
+        PairA* la = reinterpret_cast<PairA*>(p);
+        PairB* lb = reinterpret_cast<PairB*>(p + (size_t)P * sizeof(PairA));
+        for (int k = threadIdx.x; k < P; k += blockDim.x) { la[k] = gpa[k]; lb[k] = gpb[k]; }
+        sv.pa = la;
+        sv.pb = lb;
+        p += (size_t)P * (sizeof(PairA) + sizeof(PairB));
     } else {
-        sv.pairs = gpairs;
+        sv.pa = gpa;
+        sv.pb = gpb;
     }
     float4* tu = reinterpret_cast<float4*>(p);
     float4* tm = tu + T;
@@ -254,8 +259,9 @@ DEV void stage(const LigandDev* L, char* lds, SharedView& sv, TeamView& tv) {
     sv.tor_unit = tu; sv.tor_move = tm; sv.tor_atoms = ta;
     const int npad = TW * NC;
     char* w = lds + shared_bytes_for(T, P, PL) + (size_t)(threadIdx.x / TW) * team_bytes_for(npad, T);
-    tv.coords = reinterpret_cast<float4*>(w);
-    w += align16((size_t)npad * 16);
+    tv.cx = reinterpret_cast<float*>(w); w += align16((size_t)npad * 4);
+    tv.cy = reinterpret_cast<float*>(w); w += align16((size_t)npad * 4);
+    tv.cz = reinterpret_cast<float*>(w); w += align16((size_t)npad * 4);
     tv.fx = reinterpret_cast<int*>(w); w += align16((size_t)npad * 4);
     tv.fy = reinterpret_cast<int*>(w); w += align16((size_t)npad * 4);
     tv.fz = reinterpret_cast<int*>(w); w += align16((size_t)npad * 4);
@@ -305,6 +311,21 @@ struct Atoms {
     }
 };
 
+// Ligand scalars read once per kernel (kept in registers, not re-read from
+// global memory inside the loops).
+struct LigScalars {
+    int T, P, NG;
+    bool has_lig, any_general;
+    float half_smooth, elec_min, inv_elec_min, coeff_desolv;
+    DEV void load(const LigandDev* L) {
+        T = L->num_rotbonds; P = L->num_pairs; NG = L->num_genes;
+        has_lig = L->true_ligand_atoms != 0;
+        any_general = L->any_general != 0;
+        half_smooth = L->half_smooth; elec_min = L->elec_min_distance; coeff_desolv = L->coeff_desolv;
+        inv_elec_min = L->inv_elec_min;
+    }
+};
+
 template <int TW, int NC>
 struct EvalCtx {
     const LigandDev* L;
@@ -312,6 +333,7 @@ struct EvalCtx {
     SharedView sv;
     TeamView tv;
     Atoms<TW, NC> A;
+    LigScalars ls;
 };
 
 // ---------------------------------------------------------------------------
@@ -328,9 +350,9 @@ struct Pose {
 // orientation genes into pg, torsion genes into the torsion quaternions
 // (calcenergy.cl:201-221, 248-260).  Followed by wave_sync by the caller.
 template <int TW>
-DEV void gene_trig(const LigandDev* L, const SharedView& sv, const TeamView& tv, const float (&gene)[64 / TW]) {
+DEV void gene_trig(const LigScalars& ls, const SharedView& sv, const TeamView& tv, const float (&gene)[64 / TW]) {
     const int tl = Team<TW>::lane();
-    const int NG = L->num_genes;
+    const int NG = ls.NG;
 #pragma unroll
     for (int j = 0; j < Team<TW>::kGPL; ++j) {
         const int g = tl + TW * j;
@@ -400,7 +422,9 @@ DEV void compute_coords(const SharedView& sv, const TeamView& tv, const Atoms<TW
             v = make_float3(r.x + P.mv.x, r.y + P.mv.y, r.z + P.mv.z);
         }
         c[k] = v;
-        tv.coords[tl + TW * k] = make_float4(v.x, v.y, v.z, 0.f);
+        tv.cx[tl + TW * k] = v.x;
+        tv.cy[tl + TW * k] = v.y;
+        tv.cz[tl + TW * k] = v.z;
     }
 }
 
@@ -429,43 +453,38 @@ DEV void gather_issue(const GridDev& G, float3 c, uint32_t mapoff, Gather<MF>& g
     g.d0 = load_pt<MF>(G.data, nd + p); g.d1 = load_pt<MF>(G.data, nd + p + 1);
 }
 
+// The three maps are combined per cube corner first (type + q e + |q| d),
+// then interpolated once: the same sum as the reference's three separate
+// interpolations (calcenergy.cl:338-379), with one interpolation's work.
 template <int MF, bool GRAD>
 DEV float gather_finish(const GridDev& G, float3 c, float q, const Gather<MF>& g, int3& fi) {
     if (!g.in) {
         if (GRAD) { fi.x += 16777216; fi.y += 16777216; fi.z += 16777216; }
         return 16777216.0f;
     }
-    float xl = floorf(c.x), yl = floorf(c.y), zl = floorf(c.z);
-    float dx = c.x - xl, omdx = 1.0f - dx;
-    float dy = c.y - yl, omdy = 1.0f - dy;
-    float dz = c.z - zl, omdz = 1.0f - dz;
-    float w0 = omdx * omdy * omdz, w1 = omdx * dy * omdz, w2 = omdx * omdy * dz, w3 = omdx * dy * dz;
-    float w4 = dx * omdy * omdz, w5 = dx * dy * omdz, w6 = dx * omdy * dz, w7 = dx * dy * dz;
-    const float4 a0 = g.a0, a1 = g.a1, e0 = g.e0, e1 = g.e1, d0 = g.d0, d1 = g.d1;
-    float et = a0.x * w0 + a0.y * w1 + a0.z * w2 + a0.w * w3 + a1.x * w4 + a1.y * w5 + a1.z * w6 + a1.w * w7;
-    float ee = e0.x * w0 + e0.y * w1 + e0.z * w2 + e0.w * w3 + e1.x * w4 + e1.y * w5 + e1.z * w6 + e1.w * w7;
-    float ed = d0.x * w0 + d0.y * w1 + d0.z * w2 + d0.w * w3 + d1.x * w4 + d1.y * w5 + d1.z * w6 + d1.w * w7;
-    float aq = fabsf(q);
-    float e = et + q * ee + aq * ed;
+    const float xl = floorf(c.x), yl = floorf(c.y), zl = floorf(c.z);
+    const float dx = c.x - xl, omdx = 1.0f - dx;
+    const float dy = c.y - yl, omdy = 1.0f - dy;
+    const float dz = c.z - zl, omdz = 1.0f - dz;
+    const float aq = fabsf(q);
+    // cube corner k: 0 000, 1 010, 2 001, 3 011 (low x: .x .y .z .w of a0/e0/d0), 4..7 the same at x+1
+    const float4 l = make_float4(fmaf(aq, g.d0.x, fmaf(q, g.e0.x, g.a0.x)), fmaf(aq, g.d0.y, fmaf(q, g.e0.y, g.a0.y)),
+                                 fmaf(aq, g.d0.z, fmaf(q, g.e0.z, g.a0.z)), fmaf(aq, g.d0.w, fmaf(q, g.e0.w, g.a0.w)));
+    const float4 h = make_float4(fmaf(aq, g.d1.x, fmaf(q, g.e1.x, g.a1.x)), fmaf(aq, g.d1.y, fmaf(q, g.e1.y, g.a1.y)),
+                                 fmaf(aq, g.d1.z, fmaf(q, g.e1.z, g.a1.z)), fmaf(aq, g.d1.w, fmaf(q, g.e1.w, g.a1.w)));
+    // interpolate along x, then y, then z
+    const float x00 = omdx * l.x + dx * h.x, x10 = omdx * l.y + dx * h.y;
+    const float x01 = omdx * l.z + dx * h.z, x11 = omdx * l.w + dx * h.w;
+    const float y0 = omdy * x00 + dy * x10, y1 = omdy * x01 + dy * x11;
+    const float e = omdz * y0 + dz * y1;
     if (GRAD) {
         const float ig = G.inv_spacing;
-        // cube index k: 0 000, 1 010, 2 001, 3 011, 4 100, 5 110, 6 101, 7 111
-        auto gxf = [&](float4 l, float4 h) {
-            return omdz * (omdy * (h.x - l.x) + dy * (h.y - l.y)) + dz * (omdy * (h.z - l.z) + dy * (h.w - l.w));
-        };
-        auto gyf = [&](float4 l, float4 h) {
-            return omdz * (omdx * (l.y - l.x) + dx * (h.y - h.x)) + dz * (omdx * (l.w - l.z) + dx * (h.w - h.z));
-        };
-        auto gzf = [&](float4 l, float4 h) {
-            return omdy * (omdx * (l.z - l.x) + dx * (h.z - h.x)) + dy * (omdx * (l.w - l.y) + dx * (h.w - h.y));
-        };
-        float qg = q * ig, aqg = fabsf(qg);
-        float gx = gxf(a0, a1) * ig + qg * gxf(e0, e1) + aqg * gxf(d0, d1);
-        float gy = gyf(a0, a1) * ig + qg * gyf(e0, e1) + aqg * gyf(d0, d1);
-        float gz = gzf(a0, a1) * ig + qg * gzf(e0, e1) + aqg * gzf(d0, d1);
-        fi.x += quant(kTermScale * gx);
-        fi.y += quant(kTermScale * gy);
-        fi.z += quant(kTermScale * gz);
+        const float gx = omdz * (omdy * (h.x - l.x) + dy * (h.y - l.y)) + dz * (omdy * (h.z - l.z) + dy * (h.w - l.w));
+        const float gy = omdz * (x10 - x00) + dz * (x11 - x01);
+        const float gz = y1 - y0;
+        fi.x += quant(kTermScale * (gx * ig));
+        fi.y += quant(kTermScale * (gy * ig));
+        fi.z += quant(kTermScale * (gz * ig));
     }
     return e;
 }
@@ -474,40 +493,55 @@ DEV float gather_finish(const GridDev& G, float3 c, float q, const Gather<MF>& g
 // Intramolecular pair energy and dE/dr (calcenergy.cl:397-536,
 // calcMergedEneGra.cl:455-583).
 // ---------------------------------------------------------------------------
+// The cutoff tests select terms instead of branching: over a team's pairs
+// both ranges are almost always present, so branches would only add mask
+// handling.  Adding the zero of an excluded term leaves the sum unchanged.
+// rdist = 1 / (distance in grid units), inv_sp = 1 / spacing: 1/r in A is
+// their product unless r is clamped at elec_min_distance.
 template <bool GRAD>
-DEV float pair_energy(const LigandDev* L, const PairRec& pr, float ad, float& dedr) {
-    float e = 0.f;
-    dedr = 0.f;
-    if (pr.flags & (1u << 16)) { e += 50.0f * ad; dedr += 50.0f; }   // CG-G0 bond term, G = 50
-    if (ad < 8.0f) {
-        float delta = pr.reqm - ad;
-        float sd = (fabsf(delta) >= L->half_smooth) ? ad + copysignf(L->half_smooth, delta) : pr.reqm;
-        int m = (int)(pr.flags & 0xffu), n = (int)((pr.flags >> 8) & 0xffu);
-        float rm, rmn;
-        if (pr.flags & (1u << 17)) {        // m = 12, n in {6, 10}
-            float r2 = sd * sd, r4 = r2 * r2, r6 = r4 * r2;
-            rm = fast_rcp(r6 * r6);
-            rmn = (n == 6) ? r6 : r2;
-        } else {                            // general exponents (modified pairs)
-            float l2 = __builtin_amdgcn_logf(sd);
-            rm = __builtin_amdgcn_exp2f(-(float)m * l2);
-            rmn = __builtin_amdgcn_exp2f((float)(m - n) * l2);
+DEV float pair_energy(const LigScalars& ls, const PairA& pa, const PairB& pb, float ad, float rdist, float inv_sp,
+                      float& dedr) {
+    // vdW / H-bond with smoothing, r < 8 A
+    const float delta = pb.reqm - ad;
+    const float sd = (fabsf(delta) >= ls.half_smooth) ? ad + copysignf(ls.half_smooth, delta) : pb.reqm;
+    const uint32_t fl = pa.flags;
+    const float fm = (float)(fl & 0xffu), fn = (float)((fl >> 8) & 0xffu);
+    const float r2 = sd * sd, r4 = r2 * r2, r6 = r4 * r2;
+    float rm = fast_rcp(r6 * r6);                     // m = 12
+    float rmn = ((fl >> 8) & 0xffu) == 6u ? r6 : r2;  // r^(m-n), n = 6 or 10
+    if (__builtin_expect(ls.any_general, 0)) {        // other exponents (modified pairs): rare,
+        if (!(fl & (1u << 17))) {                     // kept off the common path
+            const float l2 = __builtin_amdgcn_logf(sd);
+            rm = __builtin_amdgcn_exp2f(-fm * l2);
+            rmn = __builtin_amdgcn_exp2f((fm - fn) * l2);
         }
-        e += (pr.ac - rmn * pr.bd) * rm;
-        if (GRAD) dedr += ((float)n * pr.bd * rmn - (float)m * pr.ac) * rm * fast_rcp(sd);
     }
-    if (ad < 20.48f) {
-        float d = ad < L->elec_min_distance ? L->elec_min_distance : ad;
-        float d2 = d * d;
-        float desolv = pr.dsol * (L->coeff_desolv * (12.96f - 0.1063f * d2 * (1.0f - 0.001947f * d2)) *
-                                  fast_rcp(12.96f + d2 * (0.4137f + d2 * (0.00357f + 0.000112f * d2))));
-        float ds = d + 1.26366f;
-        float rds2 = fast_rcp(ds * ds);
-        float diel = 1.10859f * rds2 + 0.010358f;
-        float rd = fast_rcp(d);
-        float es = pr.cq * rd;
-        e += diel * es + desolv;
-        if (GRAD) dedr += -es * rd * diel - es * 2.21718f * rds2 * fast_rcp(ds) - 0.0771605f * d * desolv;
+    const float evdw = (pa.ac - rmn * pa.bd) * rm;
+    // electrostatics and desolvation, r < 20.48 A, r clamped at elec_min_distance
+    const float d = ad < ls.elec_min ? ls.elec_min : ad;
+    const float d2 = d * d;
+    const float desolv = pb.dsol * (ls.coeff_desolv * (12.96f - 0.1063f * d2 * (1.0f - 0.001947f * d2)) *
+                                    fast_rcp(12.96f + d2 * (0.4137f + d2 * (0.00357f + 0.000112f * d2))));
+    const float ds = d + 1.26366f;
+    const float rds = fast_rcp(ds);
+    const float rds2 = rds * rds;
+    const float diel = 1.10859f * rds2 + 0.010358f;
+    const float rd = ad < ls.elec_min ? ls.inv_elec_min : rdist * inv_sp;
+    const float es = pb.cq * rd;
+    const float eel = diel * es + desolv;
+    const bool cgg0 = (fl & (1u << 16)) != 0u, in8 = ad < 8.0f, in20 = ad < 20.48f;
+    float e = cgg0 ? 50.0f * ad : 0.0f;               // CG-G0 bond term, G = 50
+    e += in8 ? evdw : 0.0f;
+    e += in20 ? eel : 0.0f;
+    if (GRAD) {
+        const float dvdw = (fn * pa.bd * rmn - fm * pa.ac) * rm * fast_rcp(sd);
+        const float del = -es * rd * diel - es * 2.21718f * rds2 * rds - 0.0771605f * d * desolv;
+        float g = cgg0 ? 50.0f : 0.0f;
+        g += in8 ? dvdw : 0.0f;
+        g += in20 ? del : 0.0f;
+        dedr = g;
+    } else {
+        dedr = 0.f;
     }
     return e;
 }
@@ -527,10 +561,11 @@ DEV float evaluate(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float
     const TeamView& tv = X.tv;
     const SharedView& sv = X.sv;
     const int tl = TM::lane();
-    const int T = L->num_rotbonds, P = L->num_pairs, NG = L->num_genes;
-    const bool has_lig = L->true_ligand_atoms != 0;
+    const int T = X.ls.T, P = X.ls.P, NG = X.ls.NG;
+    const bool has_lig = X.ls.has_lig;
+    (void)L;
 
-    gene_trig<TW>(L, sv, tv, gene);
+    gene_trig<TW>(X.ls, sv, tv, gene);
     if (GRAD) {
 #pragma unroll
         for (int k = 0; k < NC; ++k) { tv.fx[tl + TW * k] = 0; tv.fy[tl + TW * k] = 0; tv.fz[tl + TW * k] = 0; }
@@ -551,15 +586,16 @@ DEV float evaluate(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float
     const float spacing = G.spacing;
     float e = 0.f;
     for (int p = tl; p < P; p += TW) {
-        const PairRec pr = sv.pairs[p];
-        const int i = (int)(pr.ij & 0xffffu), j = (int)(pr.ij >> 16);
-        const float4 ci = tv.coords[i], cj = tv.coords[j];
-        const float sx = ci.x - cj.x, sy = ci.y - cj.y, sz = ci.z - cj.z;
+        const PairA pa = sv.pa[p];
+        const PairB pb = sv.pb[p];
+        const int i = (int)(pa.ij & 0xffffu), j = (int)(pa.ij >> 16);
+        const float sx = tv.cx[i] - tv.cx[j], sy = tv.cy[i] - tv.cy[j], sz = tv.cz[i] - tv.cz[j];
         const float dist = fast_sqrt(sx * sx + sy * sy + sz * sz);
+        const float rdist = fast_rcp(dist);
         float dedr;
-        e += pair_energy<GRAD>(L, pr, dist * spacing, dedr);
+        e += pair_energy<GRAD>(X.ls, pa, pb, dist * spacing, rdist, G.inv_spacing, dedr);
         if (GRAD) {
-            const float gdd = -dedr * fast_rcp(dist);
+            const float gdd = -dedr * rdist;
             const int ix = quant(kTermScale * sx * gdd), iy = quant(kTermScale * sy * gdd),
                       iz = quant(kTermScale * sz * gdd);
             atomicSub(&tv.fx[i], ix); atomicSub(&tv.fy[i], iy); atomicSub(&tv.fz[i], iz);
@@ -640,8 +676,8 @@ DEV float evaluate(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float
     // torsion genes, calcMergedEneGra.cl:866-915
     for (int t = tl; t < T; t += TW) {
         const int2 ab = sv.tor_atoms[t];
-        const float4 r0 = tv.coords[ab.x], r1 = tv.coords[ab.y];
-        const float ax = r1.x - r0.x, ay = r1.y - r0.y, az = r1.z - r0.z;
+        const float4 r0 = make_float4(tv.cx[ab.x], tv.cy[ab.x], tv.cz[ab.x], 0.f);
+        const float ax = tv.cx[ab.y] - r0.x, ay = tv.cy[ab.y] - r0.y, az = tv.cz[ab.y] - r0.z;
         const float inv = fast_rsq(ax * ax + ay * ay + az * az);
         tv.tax[t] = make_float4(ax * inv, ay * inv, az * inv, 0.f);
         tv.tref[t] = r0;
@@ -690,14 +726,15 @@ DEV void float_gradient(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], 
     const TeamView& tv = X.tv;
     const SharedView& sv = X.sv;
     const int tl = TM::lane();
-    const int T = L->num_rotbonds, P = L->num_pairs, NG = L->num_genes;
-    const bool has_lig = L->true_ligand_atoms != 0;
+    const int T = X.ls.T, P = X.ls.P, NG = X.ls.NG;
+    const bool has_lig = X.ls.has_lig;
+    (void)L;
     float* ffx = reinterpret_cast<float*>(tv.fx);
     float* ffy = reinterpret_cast<float*>(tv.fy);
     float* ffz = reinterpret_cast<float*>(tv.fz);
     float* tacc = reinterpret_cast<float*>(tv.gi);
 
-    gene_trig<TW>(L, sv, tv, gene);
+    gene_trig<TW>(X.ls, sv, tv, gene);
 #pragma unroll
     for (int k = 0; k < NC; ++k) { ffx[tl + TW * k] = 0.f; ffy[tl + TW * k] = 0.f; ffz[tl + TW * k] = 0.f; }
     for (int t = tl; t < T; t += TW) tacc[t] = 0.f;
@@ -743,36 +780,36 @@ DEV void float_gradient(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], 
         }
     }
     const float spacing = G.spacing;
-    const float half_smooth = L->half_smooth;
+    const float half_smooth = X.ls.half_smooth;
     const float DA = -8.5525f, DB = 78.4f - (-8.5525f), DK = 7.7839f;
     const float DBH = DB * 0.003627f, DBHK = DBH * DK;
     for (int p = tl; p < P; p += TW) {
-        const PairRec pr = sv.pairs[p];
-        const int i = (int)(pr.ij & 0xffffu), j = (int)(pr.ij >> 16);
-        const float4 ci = tv.coords[i], cj = tv.coords[j];
-        const float sx = ci.x - cj.x, sy = ci.y - cj.y, sz = ci.z - cj.z;
+        const PairA pa = sv.pa[p];
+        const PairB pb = sv.pb[p];
+        const int i = (int)(pa.ij & 0xffffu), j = (int)(pa.ij >> 16);
+        const float sx = tv.cx[i] - tv.cx[j], sy = tv.cy[i] - tv.cy[j], sz = tv.cz[i] - tv.cz[j];
         const float dist = sqrtf(sx * sx + sy * sy + sz * sz);
         float d = dist * spacing;
         float sd;
-        if (d <= pr.reqm - half_smooth) sd = d + half_smooth;
-        else if (d < pr.reqm + half_smooth) sd = pr.reqm;
+        if (d <= pb.reqm - half_smooth) sd = d + half_smooth;
+        else if (d < pb.reqm + half_smooth) sd = pb.reqm;
         else sd = d - half_smooth;
-        const int m = (int)(pr.flags & 0xffu), n = (int)((pr.flags >> 8) & 0xffu);
+        const int m = (int)(pa.flags & 0xffu), n = (int)((pa.flags >> 8) & 0xffu);
         float pg = 0.f;
         if (d < 8.0f) {
-            pg += (-(float)m * pr.ac) / powf(sd, (float)(m + 1));
-            pg += ((float)n * pr.bd) / powf(sd, (float)(n + 1));
+            pg += (-(float)m * pa.ac) / powf(sd, (float)(m + 1));
+            pg += ((float)n * pa.bd) / powf(sd, (float)(n + 1));
         }
         if (d < 20.48f) {
-            if (d < L->elec_min_distance) d = L->elec_min_distance;
+            if (d < X.ls.elec_min) d = X.ls.elec_min;
             const float ex = __expf(DBH * d);
             const float upper = DA * (ex + DK) * (ex + DK) + DB * ex * (DBHK * d + ex + DK);
             const float lw = DA * (ex + DK) + DB * ex;
             const float lower = d * d * lw * lw;
-            pg += -pr.cq * (upper / lower);
-            pg += pr.dsol * L->coeff_desolv * -0.077160f * d * __expf(-0.038580f * d * d);
+            pg += -pb.cq * (upper / lower);
+            pg += pb.dsol * X.ls.coeff_desolv * -0.077160f * d * __expf(-0.038580f * d * d);
         }
-        if (pr.flags & (1u << 16)) pg += 50.0f;
+        if (pa.flags & (1u << 16)) pg += 50.0f;
         const float fx = pg * (-sx / dist), fy = pg * (-sy / dist), fz = pg * (-sz / dist);
         atomicAdd(&ffx[i], -fx); atomicAdd(&ffy[i], -fy); atomicAdd(&ffz[i], -fz);
         atomicAdd(&ffx[j], fx); atomicAdd(&ffy[j], fy); atomicAdd(&ffz[j], fz);
@@ -821,8 +858,8 @@ DEV void float_gradient(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], 
     }
     for (int t = tl; t < T; t += TW) {
         const int2 ab = sv.tor_atoms[t];
-        const float4 r0 = tv.coords[ab.x], r1 = tv.coords[ab.y];
-        const float ax = r1.x - r0.x, ay = r1.y - r0.y, az = r1.z - r0.z;
+        const float4 r0 = make_float4(tv.cx[ab.x], tv.cy[ab.x], tv.cz[ab.x], 0.f);
+        const float ax = tv.cx[ab.y] - r0.x, ay = tv.cy[ab.y] - r0.y, az = tv.cz[ab.y] - r0.z;
         const float inv = rsqrtf(ax * ax + ay * ay + az * az);
         tv.tax[t] = make_float4(ax * inv, ay * inv, az * inv, 0.f);
         tv.tref[t] = r0;
@@ -880,6 +917,7 @@ struct Launch {
     X.G = a.G;                                                                 \
     stage<TW, NC, PL>(a.L, lds, X.sv, X.tv);                                   \
     X.A.load(a.L, a.G);                                                        \
+    X.ls.load(a.L);                                                            \
     const int tl = TM::lane();                                                 \
     (void)tl;                                                                  \
     const int team_idx = blockIdx.x * TM::kPerGroup + (int)(threadIdx.x / TW); \

@@ -54,24 +54,46 @@ struct adgpu_context {
     int scratch_n = 0;
 };
 
+// Runs of a job are split into groups, each with its own stream, so that the
+// generations of different groups overlap on the device (a group's local
+// search tail is filled with other groups' work).
+constexpr int kRing = 16;   // generations a group may have in flight
+
+struct RunGroup {
+    int r0 = 0, r1 = 0;                 // runs [r0, r1)
+    hipStream_t stream = nullptr;
+    int parity = 0;                     // current population in buffer A (0) or B (1)
+    int64_t done = 0, enq = 0;          // generations completed / enqueued in this adgpu_job_run
+    std::vector<int64_t> totals;        // evals of the group's runs after each completed generation
+    bool ls_used[kRing] = {};
+    hipEvent_t ev_done[kRing] = {}, ev_gen[2][kRing] = {}, ev_ls[2][kRing] = {};
+};
+
 struct adgpu_job {
     adgpu_context* ctx = nullptr;
     int map_id = 0, ligand_id = 0;
     SearchDev sd{};
     adgpu_search_params sp{};
-    JobBuffers b{};
-    int32_t* h_evals_runs = nullptr;   // pinned
+    float* pop[2] = {nullptr, nullptr};
+    float* ene[2] = {nullptr, nullptr};
+    int32_t* evals_new = nullptr;
+    int32_t* evals_runs = nullptr;
+    uint32_t* prng = nullptr;
+    int32_t* counters = nullptr;        // one LS work-queue counter per group (16 ints apart)
+    int32_t* ls_ent = nullptr;
+    float* ls_grad = nullptr;
+    int32_t* h_ring = nullptr;          // pinned [kRing][num_runs] eval totals per generation
+    std::vector<RunGroup> groups;
     int64_t generation = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double kernel_ms[2] = {0.0, 0.0};
     int64_t kernel_n[2] = {0, 0};
 };
 
 namespace {
 
-LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms) {
+LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, hipStream_t stream = nullptr) {
     LaunchArgs la{};
-    la.stream = c->stream;
+    la.stream = stream ? stream : c->stream;
     la.lig_dev = reinterpret_cast<const LigandDev*>(ls.dev);
     la.lig_host = &ls.host;
     la.grid = ms.dev;
@@ -119,6 +141,7 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
     if (NT <= 0 || NT > ADGPU_MAX_ATYPES) return fail(c, ADGPU_ERR_LIMIT, "num_atypes out of range (1..32)");
     if (P < 0 || P > N * (N - 1) / 2) return fail(c, ADGPU_ERR_LIMIT, "num_pairs out of range");
     if (L->true_ligand_atoms < 0 || L->true_ligand_atoms > N) return fail(c, ADGPU_ERR_ARG, "true_ligand_atoms out of range");
+    if (!(ep->elec_min_distance > 0.0f)) return fail(c, ADGPU_ERR_ARG, "elec_min_distance must be positive");
     if (!L->charges || !L->atom_types || !L->atom_map || !L->ref_coords || (P && !L->pairs) ||
         !L->atype_reqm || !L->vw_exp || !L->reqm_ab || !L->vw_ac || !L->vw_bd || !L->ds_s || !L->ds_v ||
         (T && (!L->rotbond_moving || !L->rotbond_unit || !L->rotbond_atoms || !L->atom_rotbonds)))
@@ -135,6 +158,7 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
     hd.qasp = ep->qasp;
     hd.half_smooth = 0.5f * ep->smooth;
     hd.elec_min_distance = ep->elec_min_distance;
+    hd.inv_elec_min = 1.0f / ep->elec_min_distance;
     size_t off = (sizeof(LigandDev) + 255) & ~size_t(255);
     auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) & ~size_t(255); return (uint32_t)o; };
     hd.off_atom_ref = take(sizeof(float4) * N);
@@ -143,7 +167,8 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
     hd.off_tor_unit = take(sizeof(float4) * (T ? T : 1));
     hd.off_tor_move = take(sizeof(float4) * (T ? T : 1));
     hd.off_tor_atoms = take(sizeof(int2) * (T ? T : 1));
-    hd.off_pairs = take(sizeof(PairRec) * (P ? P : 1));
+    hd.off_pair_a = take(sizeof(PairA) * (P ? P : 1));
+    hd.off_pair_b = take(sizeof(PairB) * (P ? P : 1));
     hd.total_bytes = (uint32_t)off;
     blob.assign(off, 0);
     float4* ar = reinterpret_cast<float4*>(blob.data() + hd.off_atom_ref);
@@ -180,10 +205,12 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
         if (dx != dy) return dx < dy;
         return std::min(L->pairs[2 * x], L->pairs[2 * x + 1]) < std::min(L->pairs[2 * y], L->pairs[2 * y + 1]);
     });
-    PairRec* pr = reinterpret_cast<PairRec*>(blob.data() + hd.off_pairs);
+    PairA* pra = reinterpret_cast<PairA*>(blob.data() + hd.off_pair_a);
+    PairB* prb = reinterpret_cast<PairB*>(blob.data() + hd.off_pair_b);
     for (int q = 0; q < P; ++q) {
         const int p = order[q];
-        PairRec& rec = pr[q];
+        PairA& ra = pra[q];
+        PairB& rb = prb[q];
         int i = L->pairs[2 * p], j = L->pairs[2 * p + 1];
         if (i < 0 || i >= N || j < 0 || j >= N) return fail(c, ADGPU_ERR_ARG, "pair atom out of range");
         int t1 = L->atom_types[i], t2 = L->atom_types[j];
@@ -192,16 +219,17 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
         uint32_t r1 = L->atype_reqm[t1], r2 = L->atype_reqm[t2];
         bool cgg0 = (r1 == 24 && r2 == 25) || (r1 == 25 && r2 == 24);   // ATYPE_CG_IDX / ATYPE_G0_IDX
         bool fast = (m == 12) && (n == 6 || n == 10);
-        rec.ij = (uint32_t)i | ((uint32_t)j << 16);
-        rec.flags = m | (n << 8) | ((uint32_t)cgg0 << 16) | ((uint32_t)fast << 17);
-        rec.ac = L->vw_ac[idx];
-        rec.bd = L->vw_bd[idx];
-        rec.reqm = L->reqm_ab[idx];
+        if (!fast) hd.any_general = 1;
+        ra.ij = (uint32_t)i | ((uint32_t)j << 16);
+        ra.flags = m | (n << 8) | ((uint32_t)cgg0 << 16) | ((uint32_t)fast << 17);
+        ra.ac = L->vw_ac[idx];
+        ra.bd = L->vw_bd[idx];
+        rb.reqm = L->reqm_ab[idx];
         float q1 = L->charges[i], q2 = L->charges[j];
-        rec.cq = ep->coeff_elec * q1 * q2;                       // calcenergy.cl:527 order
-        rec.dsol = (L->ds_s[t1] + ep->qasp * std::fabs(q1)) * L->ds_v[t2] +
-                   (L->ds_s[t2] + ep->qasp * std::fabs(q2)) * L->ds_v[t1];   // calcenergy.cl:488-491
-        rec.pad = 0.f;
+        rb.cq = ep->coeff_elec * q1 * q2;                       // calcenergy.cl:527 order
+        rb.dsol = (L->ds_s[t1] + ep->qasp * std::fabs(q1)) * L->ds_v[t2] +
+                  (L->ds_s[t2] + ep->qasp * std::fabs(q2)) * L->ds_v[t1];   // calcenergy.cl:488-491
+        rb.pad = 0.f;
     }
     std::memcpy(blob.data(), &hd, sizeof(hd));
     return ADGPU_OK;
@@ -253,6 +281,15 @@ int adgpu_set_team_width(adgpu_ctx c, int width) {
     if (width != 0 && width != 32 && width != 64) return fail(c, ADGPU_ERR_ARG, "team width must be 0, 32 or 64");
     std::lock_guard<std::mutex> lk(c->mu);
     c->team_hint = width;
+    return ADGPU_OK;
+}
+
+int adgpu_device_count(int* out_count) {
+    if (!out_count) return ADGPU_ERR_ARG;
+    *out_count = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return ADGPU_ERR_HIP;
+    *out_count = n;
     return ADGPU_OK;
 }
 
@@ -433,48 +470,71 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     j->sp = *sp;
     j->sd = to_dev(sp, li->second.host.num_genes);
     if (j->sd.num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
-    const size_t ne = (size_t)sp->num_runs * sp->pop_size;
-    bool ok = hipMalloc(&j->b.pop_cur, ne * kGenes * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->b.pop_next, ne * kGenes * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->b.ene_cur, ne * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->b.ene_next, ne * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->b.evals_new, ne * sizeof(int32_t)) == hipSuccess &&
-              hipMalloc(&j->b.evals_runs, sp->num_runs * sizeof(int32_t)) == hipSuccess &&
-              hipMalloc(&j->b.prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&j->b.ls_counter, 64 * sizeof(int32_t)) == hipSuccess &&
-              hipHostMalloc(&j->h_evals_runs, sp->num_runs * sizeof(int32_t)) == hipSuccess;
+    const int nrun = sp->num_runs;
+    const size_t ne = (size_t)nrun * sp->pop_size;
+    // run groups: up to ADGPU_RUN_GROUPS (default 3) streams
+    int ng = 3;
+    if (const char* e = std::getenv("ADGPU_RUN_GROUPS")) ng = std::atoi(e);
+    ng = std::max(1, std::min(ng, std::min(nrun, 8)));
+    bool ok = hipMalloc(&j->pop[0], ne * kGenes * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->pop[1], ne * kGenes * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->ene[0], ne * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->ene[1], ne * sizeof(float)) == hipSuccess &&
+              hipMalloc(&j->evals_new, ne * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&j->evals_runs, nrun * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&j->prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&j->counters, 16 * ng * sizeof(int32_t)) == hipSuccess &&
+              hipHostMalloc(&j->h_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess;
     if (ok && sp->ls_method == ADGPU_LS_FIRE) {
-        const size_t nls = (size_t)sp->num_runs * (j->sd.num_ls > 0 ? j->sd.num_ls : 1);
-        ok = hipMalloc(&j->b.ls_ent, nls * sizeof(int32_t)) == hipSuccess &&
-             hipMalloc(&j->b.ls_grad, nls * kGenes * sizeof(float)) == hipSuccess;
+        const size_t nls = (size_t)nrun * (j->sd.num_ls > 0 ? j->sd.num_ls : 1);
+        ok = hipMalloc(&j->ls_ent, nls * sizeof(int32_t)) == hipSuccess &&
+             hipMalloc(&j->ls_grad, nls * kGenes * sizeof(float)) == hipSuccess;
+    }
+    j->groups.resize(ng);
+    for (int k = 0; k < ng && ok; ++k) {
+        RunGroup& g = j->groups[k];
+        g.r0 = (int)((int64_t)nrun * k / ng);
+        g.r1 = (int)((int64_t)nrun * (k + 1) / ng);
+        ok = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) == hipSuccess;
+        for (int r = 0; r < kRing && ok; ++r)
+            ok = hipEventCreateWithFlags(&g.ev_done[r], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreate(&g.ev_gen[0][r]) == hipSuccess && hipEventCreate(&g.ev_gen[1][r]) == hipSuccess &&
+                 hipEventCreate(&g.ev_ls[0][r]) == hipSuccess && hipEventCreate(&g.ev_ls[1][r]) == hipSuccess;
     }
     if (!ok) {
         adgpu_job_destroy(j.release());
         return fail(c, ADGPU_ERR_NOMEM, "job allocation failed");
     }
-    for (auto& e : j->ev)
-        if (hipEventCreate(&e) != hipSuccess) { adgpu_job_destroy(j.release()); return fail(c, ADGPU_ERR_HIP, "hipEventCreate"); }
-    HIPCHK(c, hipMemcpy(j->b.pop_cur, init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(j->b.pop_next, init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(j->b.prng, seeds, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(j->b.evals_runs, 0, sp->num_runs * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(j->b.evals_new, 0, ne * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(j->b.ene_cur, 0, ne * sizeof(float)));
-    HIPCHK(c, hipMemset(j->b.ene_next, 0, ne * sizeof(float)));
-    HIPCHK(c, hipMemset(j->b.ls_counter, 0, 64 * sizeof(int32_t)));
+    HIPCHK(c, hipMemcpy(j->pop[0], init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(j->pop[1], init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(j->prng, seeds, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(j->evals_runs, 0, nrun * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(j->evals_new, 0, ne * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(j->ene[0], 0, ne * sizeof(float)));
+    HIPCHK(c, hipMemset(j->ene[1], 0, ne * sizeof(float)));
+    HIPCHK(c, hipMemset(j->counters, 0, 16 * ng * sizeof(int32_t)));
     *out = j.release();
     return ADGPU_OK;
 }
 
 int adgpu_job_destroy(adgpu_job_t j) {
     if (!j) return ADGPU_ERR_ARG;
-    if (j->ctx) { (void)hipSetDevice(j->ctx->device); (void)hipStreamSynchronize(j->ctx->stream); }
-    (void)hipFree(j->b.pop_cur); (void)hipFree(j->b.pop_next);
-    (void)hipFree(j->b.ene_cur); (void)hipFree(j->b.ene_next);
-    (void)hipFree(j->b.evals_new); (void)hipFree(j->b.evals_runs); (void)hipFree(j->b.prng);
-    (void)hipFree(j->b.ls_counter); (void)hipFree(j->b.ls_ent); (void)hipFree(j->b.ls_grad);
-    if (j->h_evals_runs) (void)hipHostFree(j->h_evals_runs);
-    for (auto& e : j->ev) if (e) (void)hipEventDestroy(e);
+    if (j->ctx) (void)hipSetDevice(j->ctx->device);
+    for (auto& g : j->groups) {
+        if (g.stream) { (void)hipStreamSynchronize(g.stream); (void)hipStreamDestroy(g.stream); }
+        for (int r = 0; r < kRing; ++r) {
+            if (g.ev_done[r]) (void)hipEventDestroy(g.ev_done[r]);
+            for (int h = 0; h < 2; ++h) {
+                if (g.ev_gen[h][r]) (void)hipEventDestroy(g.ev_gen[h][r]);
+                if (g.ev_ls[h][r]) (void)hipEventDestroy(g.ev_ls[h][r]);
+            }
+        }
+    }
+    (void)hipFree(j->pop[0]); (void)hipFree(j->pop[1]);
+    (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
+    (void)hipFree(j->evals_new); (void)hipFree(j->evals_runs); (void)hipFree(j->prng);
+    (void)hipFree(j->counters); (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
+    if (j->h_ring) (void)hipHostFree(j->h_ring);
     delete j;
     return ADGPU_OK;
 }
@@ -489,25 +549,29 @@ static int job_lookup(adgpu_job_t j, const LigandSet** ls, const MapSet** ms) {
     return ADGPU_OK;
 }
 
-int adgpu_job_initpop(adgpu_job_t j) {
-    if (!j) return ADGPU_ERR_ARG;
-    adgpu_context* c = j->ctx;
-    const LigandSet* ls; const MapSet* ms;
-    int rc = job_lookup(j, &ls, &ms);
-    if (rc) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, launch_initpop(launch_args(c, *ls, *ms), j->sd, j->b));
-    HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return ADGPU_OK;
+// Device buffers of runs [r0, r1) with the given current-population parity.
+static JobBuffers slice(const adgpu_job* j, int r0, int r1, int parity, int counter) {
+    const size_t pop = (size_t)j->sp.pop_size, off = (size_t)r0 * pop;
+    JobBuffers b{};
+    b.pop_cur = j->pop[parity] + off * kGenes;
+    b.pop_next = j->pop[parity ^ 1] + off * kGenes;
+    b.ene_cur = j->ene[parity] + off;
+    b.ene_next = j->ene[parity ^ 1] + off;
+    b.evals_new = j->evals_new + off;
+    b.evals_runs = j->evals_runs + r0;
+    b.prng = j->prng + off * ADGPU_NWI;
+    b.ls_counter = j->counters + 16 * counter;
+    const size_t nls = (size_t)(j->sd.num_ls > 0 ? j->sd.num_ls : 1);
+    b.ls_ent = j->ls_ent ? j->ls_ent + (size_t)r0 * nls : nullptr;
+    b.ls_grad = j->ls_grad ? j->ls_grad + (size_t)r0 * nls * kGenes : nullptr;
+    (void)r1;
+    return b;
 }
 
-static int run_gen(adgpu_job_t j, const LigandSet* ls, const MapSet* ms) {
-    adgpu_context* c = j->ctx;
-    HIPCHK(c, hipEventRecord(j->ev[0], c->stream));
-    HIPCHK(c, launch_gen(launch_args(c, *ls, *ms), j->sd, j->b));
-    HIPCHK(c, hipEventRecord(j->ev[1], c->stream));
-    return ADGPU_OK;
+static SearchDev slice_sd(const adgpu_job* j, int r0, int r1) {
+    SearchDev s = j->sd;
+    s.num_runs = r1 - r0;
+    return s;
 }
 
 static int ls_method_for(adgpu_job_t j, int64_t generation) {
@@ -519,48 +583,51 @@ static int ls_method_for(adgpu_job_t j, int64_t generation) {
     return m;
 }
 
-static int run_ls(adgpu_job_t j, const LigandSet* ls, const MapSet* ms, int64_t generation) {
+// Whole-job steps on the first group's stream (all groups share one parity
+// outside adgpu_job_run).
+static int whole_parity(adgpu_job_t j) { return j->groups[0].parity; }
+static void flip_all(adgpu_job_t j) { for (auto& g : j->groups) g.parity ^= 1; }
+
+int adgpu_job_initpop(adgpu_job_t j) {
+    if (!j) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
-    int m = ls_method_for(j, generation);
-    if (m == ADGPU_LS_NONE) return ADGPU_OK;
-    HIPCHK(c, hipEventRecord(j->ev[2], c->stream));
-    HIPCHK(c, launch_ls(launch_args(c, *ls, *ms), j->sd, j->b, m));
-    HIPCHK(c, hipEventRecord(j->ev[3], c->stream));
+    const LigandSet* ls; const MapSet* ms;
+    int rc = job_lookup(j, &ls, &ms);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = j->groups[0].stream;
+    const JobBuffers b = slice(j, 0, j->sp.num_runs, whole_parity(j), 0);
+    HIPCHK(c, launch_initpop(launch_args(c, *ls, *ms, s), j->sd, b));
+    HIPCHK(c, launch_sum_evals(s, j->sd, b));
+    HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;
-}
-
-static void swap_bufs(adgpu_job_t j) {
-    std::swap(j->b.pop_cur, j->b.pop_next);
-    std::swap(j->b.ene_cur, j->b.ene_next);
-}
-
-static void accumulate_times(adgpu_job_t j, bool had_ls) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, j->ev[0], j->ev[1]) == hipSuccess) { j->kernel_ms[0] += ms; j->kernel_n[0]++; }
-    if (had_ls && hipEventElapsedTime(&ms, j->ev[2], j->ev[3]) == hipSuccess) { j->kernel_ms[1] += ms; j->kernel_n[1]++; }
 }
 
 int adgpu_job_kernel_gen(adgpu_job_t j) {
     if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
     const LigandSet* ls; const MapSet* ms;
     int rc = job_lookup(j, &ls, &ms);
     if (rc) return rc;
-    HIPCHK(j->ctx, hipSetDevice(j->ctx->device));
-    rc = run_gen(j, ls, ms);
-    if (rc) return rc;
-    HIPCHK(j->ctx, hipStreamSynchronize(j->ctx->stream));
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = j->groups[0].stream;
+    HIPCHK(c, launch_gen(launch_args(c, *ls, *ms, s), j->sd, slice(j, 0, j->sp.num_runs, whole_parity(j), 0)));
+    HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;
 }
 
 int adgpu_job_kernel_ls(adgpu_job_t j, int generation) {
     if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
     const LigandSet* ls; const MapSet* ms;
     int rc = job_lookup(j, &ls, &ms);
     if (rc) return rc;
-    HIPCHK(j->ctx, hipSetDevice(j->ctx->device));
-    rc = run_ls(j, ls, ms, generation);
-    if (rc) return rc;
-    HIPCHK(j->ctx, hipStreamSynchronize(j->ctx->stream));
+    HIPCHK(c, hipSetDevice(c->device));
+    const int m = ls_method_for(j, generation);
+    if (m == ADGPU_LS_NONE) return ADGPU_OK;
+    hipStream_t s = j->groups[0].stream;
+    HIPCHK(c, launch_ls(launch_args(c, *ls, *ms, s), j->sd, slice(j, 0, j->sp.num_runs, whole_parity(j), 0), m));
+    HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;
 }
 
@@ -568,64 +635,141 @@ int adgpu_job_kernel_sum_and_swap(adgpu_job_t j) {
     if (!j) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    swap_bufs(j);
+    hipStream_t s = j->groups[0].stream;
+    HIPCHK(c, launch_sum_evals(s, j->sd, slice(j, 0, j->sp.num_runs, whole_parity(j), 0)));
+    HIPCHK(c, hipStreamSynchronize(s));
+    flip_all(j);
     j->generation++;
     return ADGPU_OK;
 }
 
 int adgpu_job_generation(adgpu_job_t j) {
     if (!j) return ADGPU_ERR_ARG;
+    int rc = adgpu_job_kernel_gen(j);
+    if (!rc) rc = adgpu_job_kernel_ls(j, (int)j->generation);
+    if (!rc) rc = adgpu_job_kernel_sum_and_swap(j);
+    return rc;
+}
+
+// Enqueues one generation of group k (K4, local search, K2, eval totals to
+// the pinned ring) on its stream; performdocking.cpp.Cuda:700-831.
+static int enqueue_generation(adgpu_job_t j, int k, const LigandSet* ls, const MapSet* ms) {
     adgpu_context* c = j->ctx;
-    const LigandSet* ls; const MapSet* ms;
-    int rc = job_lookup(j, &ls, &ms);
-    if (rc) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
-    if ((rc = run_gen(j, ls, ms))) return rc;
-    bool had_ls = ls_method_for(j, j->generation) != ADGPU_LS_NONE;
-    if ((rc = run_ls(j, ls, ms, j->generation))) return rc;
-    HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    accumulate_times(j, had_ls);
-    swap_bufs(j);
-    j->generation++;
+    RunGroup& g = j->groups[k];
+    const int slot = (int)(g.enq % kRing);
+    const LaunchArgs la = launch_args(c, *ls, *ms, g.stream);
+    const SearchDev sd = slice_sd(j, g.r0, g.r1);
+    const JobBuffers b = slice(j, g.r0, g.r1, g.parity, k);
+    const int m = ls_method_for(j, j->generation + g.enq);
+    HIPCHK(c, hipEventRecord(g.ev_gen[0][slot], g.stream));
+    HIPCHK(c, launch_gen(la, sd, b));
+    HIPCHK(c, hipEventRecord(g.ev_gen[1][slot], g.stream));
+    g.ls_used[slot] = m != ADGPU_LS_NONE;
+    if (g.ls_used[slot]) {
+        HIPCHK(c, hipEventRecord(g.ev_ls[0][slot], g.stream));
+        HIPCHK(c, launch_ls(la, sd, b, m));
+        HIPCHK(c, hipEventRecord(g.ev_ls[1][slot], g.stream));
+    }
+    HIPCHK(c, launch_sum_evals(g.stream, sd, b));
+    HIPCHK(c, hipMemcpyAsync(j->h_ring + (size_t)slot * j->sp.num_runs + g.r0, j->evals_runs + g.r0,
+                             (size_t)(g.r1 - g.r0) * sizeof(int32_t), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(c, hipEventRecord(g.ev_done[slot], g.stream));
+    g.parity ^= 1;
+    g.enq++;
+    return ADGPU_OK;
+}
+
+// Waits for the oldest in-flight generation of group k and records it.
+static int complete_generation(adgpu_job_t j, int k) {
+    adgpu_context* c = j->ctx;
+    RunGroup& g = j->groups[k];
+    const int slot = (int)(g.done % kRing);
+    HIPCHK(c, hipEventSynchronize(g.ev_done[slot]));
+    int64_t t = 0;
+    for (int r = g.r0; r < g.r1; ++r) t += j->h_ring[(size_t)slot * j->sp.num_runs + r];
+    g.totals.push_back(t);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g.ev_gen[0][slot], g.ev_gen[1][slot]) == hipSuccess) {
+        j->kernel_ms[0] += ms; j->kernel_n[0]++;
+    }
+    if (g.ls_used[slot] && hipEventElapsedTime(&ms, g.ev_ls[0][slot], g.ev_ls[1][slot]) == hipSuccess) {
+        j->kernel_ms[1] += ms; j->kernel_n[1]++;
+    }
+    g.done++;
     return ADGPU_OK;
 }
 
 // check_progress (host/src/performdocking.cpp.Cuda:908-942)
-static double progress(const adgpu_job_t j, int64_t* total) {
-    int64_t t = 0;
-    for (int r = 0; r < j->sp.num_runs; ++r) t += j->h_evals_runs[r];
-    *total = t;
-    double ep = (double)t / (double)j->sp.num_runs / (double)j->sp.max_evals * 100.0;
-    double gp = (double)j->generation / (double)j->sp.max_generations * 100.0;
+static double progress_of(const adgpu_job_t j, int64_t total, int64_t generation) {
+    double ep = (double)total / (double)j->sp.num_runs / (double)j->sp.max_evals * 100.0;
+    double gp = (double)generation / (double)j->sp.max_generations * 100.0;
     return ep > gp ? ep : gp;
 }
 
+// The generation loop of docking_with_gpu (performdocking.cpp.Cuda:685-831),
+// stopped by check_progress (:908-942) exactly as the reference: generation
+// c+1 runs iff progress after c generations is below 100 %.  Groups of runs
+// advance on their own streams; a group enqueues generation c+1 only when an
+// upper bound of the total evals after generation c (groups that have not
+// reported c yet are charged the most evals a generation can take) already
+// proves that c is not the last generation.  No group ever runs past the
+// final generation, so results equal the one-stream loop.
 int adgpu_job_run(adgpu_job_t j, int64_t* out_gens, int64_t* out_total) {
-    if (!j) return ADGPU_ERR_ARG;
+    return adgpu_job_run_generations(j, INT64_MAX, out_gens, out_total);
+}
+
+int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens, int64_t* out_total) {
+    if (!j || max_new < 0) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
     const LigandSet* ls; const MapSet* ms;
     int rc = job_lookup(j, &ls, &ms);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemcpyAsync(j->h_evals_runs, j->b.evals_runs, j->sp.num_runs * sizeof(int32_t),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int64_t total = 0;
-    while (progress(j, &total) < 100.0) {
-        if ((rc = run_gen(j, ls, ms))) return rc;
-        bool had_ls = ls_method_for(j, j->generation) != ADGPU_LS_NONE;
-        if ((rc = run_ls(j, ls, ms, j->generation))) return rc;
-        HIPCHK(c, launch_sum_evals(c->stream, j->sd, j->b));
-        HIPCHK(c, hipMemcpyAsync(j->h_evals_runs, j->b.evals_runs, j->sp.num_runs * sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        accumulate_times(j, had_ls);
-        swap_bufs(j);
-        j->generation++;
+    const int nrun = j->sp.num_runs;
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    HIPCHK(c, hipMemcpy(j->h_ring, j->evals_runs, nrun * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (auto& g : j->groups) {
+        int64_t t = 0;
+        for (int r = g.r0; r < g.r1; ++r) t += j->h_ring[r];
+        g.totals.assign(1, t);
+        g.done = g.enq = 0;
     }
+    // most evals one run can add in one generation: K4 offspring plus every LS
+    // slot at its iteration cap (two evaluations per Solis-Wets iteration)
+    const int64_t emax = (int64_t)(j->sp.pop_size - 1) +
+                         (int64_t)j->sd.num_ls * 2 * std::max<int64_t>(1, j->sp.max_num_of_iters);
+    auto bound = [&](int64_t cg) {   // upper bound of the total after cg generations
+        int64_t t = 0;
+        for (auto& g : j->groups) {
+            if (cg <= g.done) t += g.totals[cg];
+            else t += g.totals[g.done] + (cg - g.done) * (int64_t)(g.r1 - g.r0) * emax;
+        }
+        return t;
+    };
+    auto surely_continues = [&](int64_t cg) {
+        return cg < max_new && progress_of(j, bound(cg), j->generation + cg) < 100.0;
+    };
+    for (;;) {
+        for (int k = 0; k < (int)j->groups.size(); ++k) {
+            RunGroup& g = j->groups[k];
+            while (g.enq - g.done < kRing && surely_continues(g.enq))
+                if ((rc = enqueue_generation(j, k, ls, ms))) return rc;
+        }
+        int wait_k = -1;
+        for (int k = 0; k < (int)j->groups.size(); ++k) {
+            const RunGroup& g = j->groups[k];
+            if (g.enq > g.done && (wait_k < 0 || g.done < j->groups[wait_k].done)) wait_k = k;
+        }
+        if (wait_k < 0) break;   // nothing in flight and nothing may be enqueued: all stopped at the final generation
+        if ((rc = complete_generation(j, wait_k))) return rc;
+    }
+    const int64_t gens = j->groups[0].done;
+    int64_t total = 0;
+    for (auto& g : j->groups) {
+        if (g.done != gens) return fail(c, ADGPU_ERR_STATE, "run groups stopped at different generations");
+        total += g.totals[gens];
+    }
+    j->generation += gens;
     if (out_gens) *out_gens = j->generation;
     if (out_total) *out_total = total;
     return ADGPU_OK;
@@ -635,12 +779,17 @@ int adgpu_job_download(adgpu_job_t j, float* pop, float* ene, int32_t* evals_run
     if (!j) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const size_t ne = (size_t)j->sp.num_runs * j->sp.pop_size;
-    if (pop) HIPCHK(c, hipMemcpy(pop, j->b.pop_cur, ne * kGenes * sizeof(float), hipMemcpyDeviceToHost));
-    if (ene) HIPCHK(c, hipMemcpy(ene, j->b.ene_cur, ne * sizeof(float), hipMemcpyDeviceToHost));
-    if (evals_runs) HIPCHK(c, hipMemcpy(evals_runs, j->b.evals_runs, j->sp.num_runs * sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (prng) HIPCHK(c, hipMemcpy(prng, j->b.prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    const size_t pz = (size_t)j->sp.pop_size;
+    for (auto& g : j->groups) {   // each group's current buffer
+        const size_t off = (size_t)g.r0 * pz, n = (size_t)(g.r1 - g.r0) * pz;
+        if (pop) HIPCHK(c, hipMemcpy(pop + off * kGenes, j->pop[g.parity] + off * kGenes, n * kGenes * sizeof(float),
+                                     hipMemcpyDeviceToHost));
+        if (ene) HIPCHK(c, hipMemcpy(ene + off, j->ene[g.parity] + off, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    const size_t ne = (size_t)j->sp.num_runs * pz;
+    if (evals_runs) HIPCHK(c, hipMemcpy(evals_runs, j->evals_runs, j->sp.num_runs * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (prng) HIPCHK(c, hipMemcpy(prng, j->prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return ADGPU_OK;
 }
 
@@ -648,11 +797,16 @@ int adgpu_job_upload(adgpu_job_t j, const float* pop, const float* ene, const ui
     if (!j) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const size_t ne = (size_t)j->sp.num_runs * j->sp.pop_size;
-    if (pop) HIPCHK(c, hipMemcpy(j->b.pop_cur, pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
-    if (ene) HIPCHK(c, hipMemcpy(j->b.ene_cur, ene, ne * sizeof(float), hipMemcpyHostToDevice));
-    if (prng) HIPCHK(c, hipMemcpy(j->b.prng, prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    const size_t pz = (size_t)j->sp.pop_size;
+    for (auto& g : j->groups) {
+        const size_t off = (size_t)g.r0 * pz, n = (size_t)(g.r1 - g.r0) * pz;
+        if (pop) HIPCHK(c, hipMemcpy(j->pop[g.parity] + off * kGenes, pop + off * kGenes, n * kGenes * sizeof(float),
+                                     hipMemcpyHostToDevice));
+        if (ene) HIPCHK(c, hipMemcpy(j->ene[g.parity] + off, ene + off, n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    const size_t ne = (size_t)j->sp.num_runs * pz;
+    if (prng) HIPCHK(c, hipMemcpy(j->prng, prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
     return ADGPU_OK;
 }
 

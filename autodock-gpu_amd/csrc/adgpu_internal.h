@@ -14,12 +14,16 @@ constexpr int kGroupThreads = 256;   // threads per workgroup (its teams share t
 constexpr int kGenes = ADGPU_GENOTYPE_STRIDE;
 
 // One intramolecular pair, precomputed on the host from the reference's
-// per-type tables (the lookups of calcenergy.cl:397-536 folded per pair).
-struct __attribute__((aligned(16))) PairRec {
+// per-type tables (the lookups of calcenergy.cl:397-536 folded per pair), in
+// two 16-byte halves stored as separate arrays so that consecutive lanes read
+// consecutive 16-byte records (no LDS bank conflicts).
+struct __attribute__((aligned(16))) PairA {
     uint32_t ij;      // atom1 | atom2 << 16
     uint32_t flags;   // m | n << 8 | (CG-G0 bond) << 16 | (fast 12-6 / 12-10 form) << 17
     float ac;         // VWpars_AC of the type pair
     float bd;         // VWpars_BD
+};
+struct __attribute__((aligned(16))) PairB {
     float reqm;       // reqm_AB (optimum distance)
     float cq;         // coeff_elec * q1 * q2 (same operation order as the reference)
     float dsol;       // (S1 + qasp|q1|) V2 + (S2 + qasp|q2|) V1
@@ -30,17 +34,18 @@ struct __attribute__((aligned(16))) PairRec {
 // Offsets are in bytes from the start of the block.
 struct LigandDev {
     int32_t num_atoms, true_ligand_atoms, num_rotbonds, num_genes;
-    int32_t num_pairs, nchunk, pad0, pad1;
+    int32_t num_pairs, nchunk, any_general, pad1;   // any_general: some pair is not 12-6 / 12-10
     float coeff_elec, coeff_desolv, qasp, half_smooth;
-    float elec_min_distance, pad2, pad3, pad4;
+    float elec_min_distance, inv_elec_min, pad3, pad4;
     uint32_t off_atom_ref;    // float4 [N]  x, y, z (grid units), charge
     uint32_t off_atom_info;   // int4   [N]  map index, type, ignore_inter, 0
     uint32_t off_atom_mask;   // uint2  [N]  bit t = moves with rotatable bond t
     uint32_t off_tor_unit;    // float4 [T]  unit vector
     uint32_t off_tor_move;    // float4 [T]  moving vector
     uint32_t off_tor_atoms;   // int2   [T]  bond atoms
-    uint32_t off_pairs;       // PairRec[P]
-    uint32_t total_bytes;
+    uint32_t off_pair_a;      // PairA[P]
+    uint32_t off_pair_b;      // PairB[P]
+    uint32_t total_bytes, pad5, pad6, pad7;
 };
 
 struct GridDev {

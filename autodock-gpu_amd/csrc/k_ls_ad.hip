@@ -56,6 +56,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
             const bool better = e < best_e;
 #pragma unroll
             for (int j = 0; j < GPL; ++j) {
+                if (j > 0 && NG <= TW) break;   // genes beyond the first lane set are all zero
                 if (better) best[j] = geno[j];
                 sqg[j] *= RHO;
                 sqg[j] += (1.0f - RHO) * g[j] * g[j];

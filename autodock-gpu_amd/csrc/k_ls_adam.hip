@@ -59,6 +59,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_adam(Launch a) {
             const bool better = e < best_e;
 #pragma unroll
             for (int j = 0; j < GPL; ++j) {
+                if (j > 0 && NG <= TW) break;   // genes beyond the first lane set are all zero
                 if (better) best[j] = geno[j];
                 mt[j] = b1 * mt[j] + (1.0f - b1) * g[j];
                 vt[j] = b2 * vt[j] + (1.0f - b2) * g[j] * g[j];

@@ -81,6 +81,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_sw(Launch a) {
         if (!done && phase == 0) {
 #pragma unroll
             for (int j = 0; j < GPL; ++j) {
+                if (j > 0 && NG <= TW) break;   // genes beyond the first lane set are inactive
                 if (!active[j]) continue;
                 const int g = tl + TW * j;
                 const float r1 = randf(state[j]);

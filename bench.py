@@ -41,7 +41,24 @@ def parse():
     ap.add_argument("--maps", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--team-width", type=int, default=0, choices=[0, 32, 64])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one 1 GiB device-to-device copy kernel (the reference point for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """One plain copy kernel of known size: 2^28 floats (1 GiB) read and written.
+    Under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE give the counters' scale."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
 
 
 def build_inputs():
@@ -105,6 +122,9 @@ def cpu_baseline(maps, tables, ep, sp_gpu, seconds):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one, and the docking "
               "path shards by ligand/run with no inter-GPU exchange (see the sharding note "
@@ -114,6 +134,8 @@ def main():
     lig, maps, tables, ep = build_inputs()
     fmt = {"f32": adgpu.MAP_F32, "f16": adgpu.MAP_F16, "bf16": adgpu.MAP_BF16}[a.maps]
     ctx = adgpu.Context(0)
+    if a.team_width:
+        ctx.set_team_width(a.team_width)
     mid = ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing, fmt)
     lid = ctx.ligand_upload(tables, ep)
     sp = adgpu.search_params(num_runs=a.nrun, pop_size=a.pop, max_evals=a.evals, ls=a.ls, spacing=maps.spacing)

@@ -147,6 +147,10 @@ typedef struct adgpu_search_params {
 typedef struct adgpu_context* adgpu_ctx;
 typedef struct adgpu_job*     adgpu_job_t;
 
+/* Number of usable GPUs (replaces the device enumeration of get_gpu_pool,
+ * host/src/performdocking.cpp.Cuda:96-115). */
+int  adgpu_device_count(int* out_count);
+
 /* --- context ---------------------------------------------------------------
  * Replaces setup_gpu_for_docking / finish_gpu_from_docking
  * (host/inc/performdocking.h.Cuda:65-75, host/src/performdocking.cpp.Cuda:117-222). */
@@ -211,6 +215,12 @@ int  adgpu_job_generation(adgpu_job_t job);
 /* Runs generations until check_progress reaches 100 % (host/src/performdocking.cpp.Cuda:
  * 685, 908-942).  Returns the number of generations in *out_generations. */
 int  adgpu_job_run(adgpu_job_t job, int64_t* out_generations, int64_t* out_total_evals);
+/* As adgpu_job_run, but runs at most max_new_generations generations in this
+ * call, so a caller can stop between generations (the AutoStop test every
+ * as_frequency generations, performdocking.cpp.Cuda:687-696).  The loop has
+ * finished when fewer generations than asked for were run. */
+int  adgpu_job_run_generations(adgpu_job_t job, int64_t max_new_generations,
+                               int64_t* out_generations, int64_t* out_total_evals);
 /* Host copies of the current population [runs*pop*64], energies [runs*pop],
  * per-run eval totals [runs] and LCG states [runs*pop*64]; any may be NULL.
  * Mirrors the final downloads at host/src/performdocking.cpp.Cuda:856-859. */
