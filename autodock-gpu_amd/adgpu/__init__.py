@@ -52,6 +52,17 @@ class Context:
             msg = self.lib.adgpu_last_error(self.h)
             raise AdgpuError("%s failed (status %d): %s" % (what, rc, msg.decode() if msg else ""))
 
+    def rescore_f64(self, map_id, lig_id, genotypes):
+        """FP64 total, inter and intra energies of host genotypes on the GPU."""
+        g = _f32(genotypes).reshape(-1, GENOTYPE_STRIDE)
+        n = g.shape[0]
+        tot, inter, intra = np.zeros(n), np.zeros(n), np.zeros(n)
+        dp = C.POINTER(C.c_double)
+        self._check(self.lib.adgpu_rescore_f64(self.h, map_id, lig_id, g.ctypes.data_as(_fp), n,
+                                               tot.ctypes.data_as(dp), inter.ctypes.data_as(dp),
+                                               intra.ctypes.data_as(dp)), "adgpu_rescore_f64")
+        return tot, inter, intra
+
     def set_team_width(self, width: int):
         """0 = per ligand, 32 = two individuals per wavefront, 64 = one."""
         self._check(self.lib.adgpu_set_team_width(self.h, int(width)), "adgpu_set_team_width")
@@ -162,6 +173,20 @@ class Job:
             self._c(self.lib.adgpu_job_run_generations(self.h, int(max_new_generations), C.byref(g), C.byref(t)),
                     "adgpu_job_run_generations")
         return g.value, t.value
+
+    def best_f64(self):
+        """Each run's best individual and its FP64 re-scored energies."""
+        nr = self.sp.num_runs
+        geno = np.zeros((nr, GENOTYPE_STRIDE), np.float32)
+        e32 = np.zeros(nr, np.float32)
+        idx = np.zeros(nr, np.int32)
+        tot, inter, intra = np.zeros(nr), np.zeros(nr), np.zeros(nr)
+        dp = C.POINTER(C.c_double)
+        self._c(self.lib.adgpu_job_best_f64(self.h, geno.ctypes.data_as(_fp), e32.ctypes.data_as(_fp),
+                                            idx.ctypes.data_as(_ip), tot.ctypes.data_as(dp),
+                                            inter.ctypes.data_as(dp), intra.ctypes.data_as(dp)),
+                "adgpu_job_best_f64")
+        return dict(genotypes=geno, energy_f32=e32, index=idx, total=tot, inter=inter, intra=intra)
 
     def download(self):
         pop = np.zeros((self.ne, GENOTYPE_STRIDE), np.float32)

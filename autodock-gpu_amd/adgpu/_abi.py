@@ -172,6 +172,9 @@ def load_library(path: str | None = None):
     lib.adgpu_job_run.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.adgpu_job_run_generations.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.adgpu_device_count.argtypes = [C.POINTER(C.c_int)]
+    _dp = C.POINTER(C.c_double)
+    lib.adgpu_rescore_f64.argtypes = [vp, C.c_int, C.c_int, _fp, C.c_int, _dp, _dp, _dp]
+    lib.adgpu_job_best_f64.argtypes = [vp, _fp, _fp, _ip, _dp, _dp, _dp]
     lib.adgpu_job_download.argtypes = [vp, _fp, _fp, _ip, _up]
     lib.adgpu_job_upload.argtypes = [vp, _fp, _fp, _up]
     lib.adgpu_job_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -180,7 +183,7 @@ def load_library(path: str | None = None):
     return lib
 
 
-EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "adgpu_set_team_width", "adgpu_device_count", "adgpu_job_run_generations", "adgpu_maps_upload",
+EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "adgpu_set_team_width", "adgpu_device_count", "adgpu_job_run_generations", "adgpu_rescore_f64", "adgpu_job_best_f64", "adgpu_maps_upload",
             "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses",
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",
             "adgpu_job_kernel_ls", "adgpu_job_kernel_sum_and_swap", "adgpu_job_generation", "adgpu_job_run",

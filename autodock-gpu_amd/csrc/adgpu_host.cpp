@@ -52,6 +52,8 @@ struct adgpu_context {
     float* d_ene = nullptr;
     float* d_grad = nullptr;
     int scratch_n = 0;
+    double* d_f64 = nullptr;   // FP64 re-scoring results [3n]
+    int f64_n = 0;
 };
 
 // Runs of a job are split into groups, each with its own stream, so that the
@@ -91,9 +93,11 @@ struct adgpu_job {
 
 namespace {
 
-LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, hipStream_t stream = nullptr) {
+LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, hipStream_t stream = nullptr,
+                       int share = 1) {
     LaunchArgs la{};
     la.stream = stream ? stream : c->stream;
+    la.share = share;
     la.lig_dev = reinterpret_cast<const LigandDev*>(ls.dev);
     la.lig_host = &ls.host;
     la.grid = ms.dev;
@@ -169,6 +173,7 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
     hd.off_tor_atoms = take(sizeof(int2) * (T ? T : 1));
     hd.off_pair_a = take(sizeof(PairA) * (P ? P : 1));
     hd.off_pair_b = take(sizeof(PairB) * (P ? P : 1));
+    hd.off_type_ds = take(sizeof(float2) * NT);
     hd.total_bytes = (uint32_t)off;
     blob.assign(off, 0);
     float4* ar = reinterpret_cast<float4*>(blob.data() + hd.off_atom_ref);
@@ -205,6 +210,8 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
         if (dx != dy) return dx < dy;
         return std::min(L->pairs[2 * x], L->pairs[2 * x + 1]) < std::min(L->pairs[2 * y], L->pairs[2 * y + 1]);
     });
+    float2* tds = reinterpret_cast<float2*>(blob.data() + hd.off_type_ds);
+    for (int t = 0; t < NT; ++t) tds[t] = make_float2(L->ds_s[t], L->ds_v[t]);
     PairA* pra = reinterpret_cast<PairA*>(blob.data() + hd.off_pair_a);
     PairB* prb = reinterpret_cast<PairB*>(blob.data() + hd.off_pair_b);
     for (int q = 0; q < P; ++q) {
@@ -321,6 +328,7 @@ int adgpu_close(adgpu_ctx c) {
     (void)hipFree(c->d_geno);
     (void)hipFree(c->d_ene);
     (void)hipFree(c->d_grad);
+    (void)hipFree(c->d_f64);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return ADGPU_OK;
@@ -450,6 +458,48 @@ int adgpu_eval_poses(adgpu_ctx c, int map_id, int lig_id, int mode, const float*
     return ADGPU_OK;
 }
 
+int adgpu_rescore_f64(adgpu_ctx c, int map_id, int lig_id, const float* genotypes, int n, double* total,
+                      double* inter, double* intra) {
+    if (!c || !genotypes || n < 0) return fail(c, ADGPU_ERR_ARG, "null argument");
+    if (n == 0) return ADGPU_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto mi = c->maps.find(map_id);
+    auto li = c->ligands.find(lig_id);
+    if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_ARG, "unknown map or ligand id");
+    if (li->second.max_map >= mi->second.dev.nmaps - 2)
+        return fail(c, ADGPU_ERR_ARG, "ligand atom_map index exceeds the grid's type maps");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n > c->scratch_n) {
+        (void)hipFree(c->d_geno); (void)hipFree(c->d_ene); (void)hipFree(c->d_grad);
+        c->d_geno = c->d_ene = c->d_grad = nullptr;
+        c->scratch_n = 0;
+        if (hipMalloc(&c->d_geno, sizeof(float) * kGenes * (size_t)n) != hipSuccess ||
+            hipMalloc(&c->d_ene, sizeof(float) * (size_t)n) != hipSuccess ||
+            hipMalloc(&c->d_grad, sizeof(float) * kGenes * (size_t)n) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(eval scratch) failed");
+        c->scratch_n = n;
+    }
+    if (n > c->f64_n) {
+        (void)hipFree(c->d_f64);
+        c->d_f64 = nullptr;
+        c->f64_n = 0;
+        if (hipMalloc(&c->d_f64, sizeof(double) * 3 * (size_t)n) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(FP64 scratch) failed");
+        c->f64_n = n;
+    }
+    std::vector<double> h(3 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(c->d_geno, genotypes, sizeof(float) * kGenes * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_rescore_f64(launch_args(c, li->second, mi->second), c->d_geno, n, c->d_f64));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_f64, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        if (total) total[i] = h[3 * (size_t)i];
+        if (inter) inter[i] = h[3 * (size_t)i + 1];
+        if (intra) intra[i] = h[3 * (size_t)i + 2];
+    }
+    return ADGPU_OK;
+}
+
 int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_params* sp, const float* init_pop,
                      const uint32_t* seeds, adgpu_job_t* out) {
     if (!c || !sp || !init_pop || !seeds || !out) return fail(c, ADGPU_ERR_ARG, "null argument");
@@ -472,8 +522,10 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     if (j->sd.num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
-    // run groups: up to ADGPU_RUN_GROUPS (default 3) streams
-    int ng = 3;
+    // run groups: ADGPU_RUN_GROUPS streams (default 2: with the context's
+    // stream that is 3 of HIP's 4 hardware queues per process; groups whose
+    // streams share a hardware queue would serialize)
+    int ng = 2;
     if (const char* e = std::getenv("ADGPU_RUN_GROUPS")) ng = std::atoi(e);
     ng = std::max(1, std::min(ng, std::min(nrun, 8)));
     bool ok = hipMalloc(&j->pop[0], ne * kGenes * sizeof(float)) == hipSuccess &&
@@ -657,7 +709,7 @@ static int enqueue_generation(adgpu_job_t j, int k, const LigandSet* ls, const M
     adgpu_context* c = j->ctx;
     RunGroup& g = j->groups[k];
     const int slot = (int)(g.enq % kRing);
-    const LaunchArgs la = launch_args(c, *ls, *ms, g.stream);
+    const LaunchArgs la = launch_args(c, *ls, *ms, g.stream, (int)j->groups.size());
     const SearchDev sd = slice_sd(j, g.r0, g.r1);
     const JobBuffers b = slice(j, g.r0, g.r1, g.parity, k);
     const int m = ls_method_for(j, j->generation + g.enq);
@@ -808,6 +860,27 @@ int adgpu_job_upload(adgpu_job_t j, const float* pop, const float* ene, const ui
     const size_t ne = (size_t)j->sp.num_runs * pz;
     if (prng) HIPCHK(c, hipMemcpy(j->prng, prng, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
     return ADGPU_OK;
+}
+
+int adgpu_job_best_f64(adgpu_job_t j, float* best_geno, float* best_e, int32_t* best_idx, double* total,
+                       double* inter, double* intra) {
+    if (!j) return ADGPU_ERR_ARG;
+    const int nrun = j->sp.num_runs, pz = j->sp.pop_size;
+    std::vector<float> pop((size_t)nrun * pz * kGenes), ene((size_t)nrun * pz);
+    int rc = adgpu_job_download(j, pop.data(), ene.data(), nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<float> geno((size_t)nrun * kGenes, 0.0f);
+    for (int r = 0; r < nrun; ++r) {
+        int bi = 0;
+        for (int k = 1; k < pz; ++k)
+            if (ene[(size_t)r * pz + k] < ene[(size_t)r * pz + bi]) bi = k;
+        std::memcpy(&geno[(size_t)r * kGenes], &pop[((size_t)r * pz + bi) * kGenes], kGenes * sizeof(float));
+        if (best_idx) best_idx[r] = bi;
+        if (best_e) best_e[r] = ene[(size_t)r * pz + bi];
+    }
+    if (best_geno) std::memcpy(best_geno, geno.data(), geno.size() * sizeof(float));
+    if (!total && !inter && !intra) return ADGPU_OK;
+    return adgpu_rescore_f64(j->ctx, j->map_id, j->ligand_id, geno.data(), nrun, total, inter, intra);
 }
 
 int adgpu_job_kernel_time(adgpu_job_t j, int cls, double* avg_ms, int64_t* launches) {

@@ -45,7 +45,8 @@ struct LigandDev {
     uint32_t off_tor_atoms;   // int2   [T]  bond atoms
     uint32_t off_pair_a;      // PairA[P]
     uint32_t off_pair_b;      // PairB[P]
-    uint32_t total_bytes, pad5, pad6, pad7;
+    uint32_t off_type_ds;     // float2 [num_atypes] (dspars_S, dspars_V) per ligand type (FP64 re-scoring)
+    uint32_t total_bytes, pad6, pad7;
 };
 
 struct GridDev {
@@ -86,9 +87,13 @@ struct LaunchArgs {
     const LigandDev* lig_host;   // host copy of its header
     GridDev grid;
     int team_hint;               // 0 = automatic, 32 or 64 = team width
+    int share;                   // concurrent launches sharing the device (run groups); persistent
+                                 // local-search grids take 1/share of the resident capacity
 };
 
 // Launchers (k_*.hip).
+// FP64 energies of n poses (k_rescore.hip): total, inter and intra per pose.
+hipError_t launch_rescore_f64(const LaunchArgs& la, const float* genotypes, int n, double* out3);
 hipError_t launch_eval_poses(const LaunchArgs& la, int mode, const float* genotypes, int n, float* energies,
                              float* grads);
 hipError_t launch_initpop(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);

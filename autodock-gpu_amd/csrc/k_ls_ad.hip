@@ -94,7 +94,7 @@ hipError_t launch_ls_ad(const LaunchArgs& la, const SearchDev& sp, const JobBuff
     hipError_t err = hipSuccess;
     ADGPU_DISPATCH(k_ls_ad, sh, la.grid.format, {
         const int want = (n + per_group - 1) / per_group;
-        const int cap = resident_groups(reinterpret_cast<const void*>(kfn), sh.lds);
+        const int cap = resident_groups(reinterpret_cast<const void*>(kfn), sh.lds) / (la.share > 1 ? la.share : 1);
         const int groups = cap > 0 && cap < want ? cap : want;
         hipLaunchKernelGGL(kfn, dim3(groups), dim3(256), sh.lds, la.stream, a);
         err = hipGetLastError();

@@ -157,7 +157,7 @@ hipError_t launch_ls_fire(const LaunchArgs& la, const SearchDev& sp, const JobBu
     });
     if (err != hipSuccess) return err;
     ADGPU_DISPATCH(k_ls_fire, sh, la.grid.format, {
-        const int cap = resident_groups(reinterpret_cast<const void*>(kfn), sh.lds);
+        const int cap = resident_groups(reinterpret_cast<const void*>(kfn), sh.lds) / (la.share > 1 ? la.share : 1);
         const int groups = cap > 0 && cap < want ? cap : want;
         hipLaunchKernelGGL(kfn, dim3(groups), dim3(256), sh.lds, la.stream, a);
         err = hipGetLastError();

@@ -28,6 +28,14 @@ sys.path.insert(0, os.path.join(HERE, "autodock-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 
+# Memory traffic (L2 <-> memory) of the local-search kernel per local-search
+# evaluation, from two rocprofv3 --pmc runs of this workload (FETCH_SIZE, then
+# WRITE_SIZE) scaled by the same counters read on a 1 GiB device-to-device
+# copy kernel (FETCH_SIZE reports 0.5 GiB for the 1 GiB read: factor 2.0;
+# WRITE_SIZE reports 1 GiB: factor 1.0).  Summary and commands:
+# profiles/k_ls_ad_c1_traffic.md.  Keyed by (maps, nrun, pop, ls).
+TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 1178.0}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -178,6 +186,13 @@ def main():
     avg_ls_ms = tot_ls_ms / max(1, int(np.sum(ls_n)))
     bytes_per_launch = ls_evals * bytes_per_eval / max(1, int(np.sum(ls_n)))
     achieved = bytes_per_launch / (avg_ls_ms * 1e-3) / 1e9
+    tkey = (a.maps, a.nrun, a.pop, a.ls)
+    traffic = (TRAFFIC_BYTES_PER_LS_EVAL[tkey] * ls_evals / max(1, int(np.sum(ls_n)))
+               if tkey in TRAFFIC_BYTES_PER_LS_EVAL else None)
+    # the run groups' launches overlap on the device, so per-launch durations
+    # include co-running work; the whole-job rate over the generation loop is
+    # reported beside it
+    aggregate = ls_evals * bytes_per_eval / t / 1e9
     out = {
         "metric": "LGA energy evals/sec + ligands/hour at 2.5M evals; p50/p99 ligand latency",
         "value": evals_per_s,
@@ -200,8 +215,9 @@ def main():
         "kernels": {"ls_avg_ms": avg_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
                     "ls_launches": int(np.sum(ls_n)), "ls_share_of_wall": tot_ls_ms / 1000.0 / t},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_eval": bytes_per_eval},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_eval": bytes_per_eval, "aggregate_achieved": aggregate,
+                     "aggregate_frac": aggregate / HBM_PEAK_GBS},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(maps, tables, ep, sp, a.cpu_seconds)

@@ -185,6 +185,14 @@ int  adgpu_ligand_release(adgpu_ctx ctx, int ligand_id);
 int  adgpu_eval_poses(adgpu_ctx ctx, int map_id, int ligand_id, int mode,
                       const float* genotypes, int n, float* energies, float* grads);
 
+/* FP64 energies of n host genotypes on the GPU: total, inter and intra per
+ * pose (any output may be NULL).  Same terms as gpu_calc_energy
+ * (device/calcenergy.cl:132-554) evaluated in double; the final re-scoring
+ * step that the reference does on the CPU in FP32 (process_result,
+ * host/src/processresult.cpp:430-466) for each run's best pose. */
+int  adgpu_rescore_f64(adgpu_ctx ctx, int map_id, int ligand_id, const float* genotypes, int n,
+                       double* total, double* inter, double* intra);
+
 /* --- docking jobs --------------------------------------------------------------
  * A job owns the device state docking_with_gpu allocates per ligand
  * (host/src/performdocking.cpp.Cuda:325-342): two population/energy buffers,
@@ -229,6 +237,11 @@ int  adgpu_job_download(adgpu_job_t job, float* population, float* energies,
 /* Overwrites the current population, energies and LCG states (any may be NULL). */
 int  adgpu_job_upload(adgpu_job_t job, const float* population, const float* energies,
                       const uint32_t* prng_states);
+/* Best individual of each run of the current population (lowest FP32
+ * energy, first index on ties) and its FP64 re-scored energies; each array
+ * has num_runs entries (genotypes: num_runs*64 floats); any may be NULL. */
+int  adgpu_job_best_f64(adgpu_job_t job, float* best_genotypes, float* best_energies, int32_t* best_index,
+                        double* total, double* inter, double* intra);
 /* Average device time in milliseconds of the last launches of one kernel
  * class, measured with HIP events on the job's stream (0 = gen, 1 = ls). */
 int  adgpu_job_kernel_time(adgpu_job_t job, int kernel_class, double* avg_ms, int64_t* launches);

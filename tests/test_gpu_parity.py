@@ -208,3 +208,42 @@ def test_team_width_parity(gpu_ctx, width):
         d = np.abs(g[i, :ng] - go)
         viol += int(np.sum(d[well] > (1e-4 * np.abs(go) + 4 * qv)[well]))
     assert viol <= 0.002 * len(poses) * len(well), viol
+
+
+@pytest.mark.parametrize("name", ["c1", "a90_t20"])
+def test_fp64_rescoring_matches_fp64_oracle(gpu_ctx, name):
+    """FP64 re-scoring on the GPU equals the FP64 oracle to 1e-10 of the sum of
+    |terms| (double rounding plus the device's and libm's last-ulp sin/cos/pow
+    differences); inter and intra separately."""
+    kw = _cases()[name]
+    case = helpers.synthetic_case(kw["n_atoms"], kw["n_tors"], kw["lig_seed"], kw["map_seed"], npts=kw["npts"])
+    mid, lid = _upload(gpu_ctx, case)
+    poses = helpers.poses_for(case, 300, seed=606)
+    tot, inter, intra = gpu_ctx.rescore_f64(mid, lid, poses)
+    for i, p in enumerate(poses):
+        e, ei, ea, s = case.orc.energy64(p)
+        tol = 1e-10 * max(1.0, s)
+        assert abs(tot[i] - e) <= tol, (i, tot[i], e)
+        assert abs(inter[i] - ei) <= tol and abs(intra[i] - ea) <= tol, (i, inter[i], ei, intra[i], ea)
+
+
+def test_job_best_f64(gpu_ctx):
+    """Each run's best individual of a short LGA job, re-scored in FP64: the
+    index is the run's lowest FP32 energy and the FP64 energy is the oracle's."""
+    case = helpers.synthetic_case(30, 6, 7, 11, npts=60)
+    mid, lid = _upload(gpu_ctx, case)
+    sp = search_params(num_runs=5, pop_size=30, max_evals=20000, ls="ad", spacing=case.spacing)
+    init = O.initial_population((4, 4), 5, 30, case.maps.size_angstr, case.spacing)
+    job = gpu_ctx.job(mid, lid, sp, init, O.cmwc_uints((4, 4), 5 * 30 * 64))
+    job.initpop()
+    job.run()
+    st = job.download()
+    b = job.best_f64()
+    ene = st["ene"].reshape(5, 30)
+    assert np.array_equal(b["index"], np.argmin(ene, axis=1))
+    for r in range(5):
+        e64, _, _, s = case.orc.energy64(b["genotypes"][r])
+        assert abs(b["total"][r] - e64) <= 1e-10 * max(1.0, s)
+        # the FP32 search energy of the same pose is within the FP32 bar of its FP64 value
+        assert abs(b["energy_f32"][r] - e64) <= 1e-4 * max(1.0, s) + 1e-3
+    job.destroy()
