@@ -615,6 +615,37 @@ def write_maps(maps: Maps, directory: str, base: str = "synth") -> str:
     return path
 
 
+def make_flexres(num_atoms: int = 12, num_torsions: int = 4, seed: int = 0, offset=(0.0, 0.0, 0.0),
+                 charge_range: float = 0.3, resname: str = "FLX", resnum: int = 1) -> str:
+    """One flexible residue as the reference's flexres pdbqt reader takes it
+    (parse_liganddata, processligand.cpp:1124-1300: each ROOT starts a
+    residue; its first two atoms get ignore_inter): a synthetic tree of
+    `num_atoms` atoms with `num_torsions` rotatable bonds, translated by
+    `offset` (A, receptor frame)."""
+    lig = make_ligand(num_atoms, num_torsions, seed=seed, charge_range=charge_range)
+    out = ["BEGIN_RES %s A%4d" % (resname, resnum)]
+    for line in lig.pdbqt_lines:
+        if line.startswith(("REMARK", "TORSDOF")):
+            continue
+        if line.startswith("HETATM"):
+            x, y, z = float(line[30:38]) + offset[0], float(line[38:46]) + offset[1], float(line[46:54]) + offset[2]
+            line = "ATOM  " + line[6:17] + "%3s A%4d    %8.3f%8.3f%8.3f" % (resname, resnum, x, y, z) + line[54:]
+        out.append(line)
+    out.append("END_RES %s A%4d" % (resname, resnum))
+    return "\n".join(out) + "\n"
+
+
+def flexres_types(text: str) -> list:
+    """AD4 atom types of a flexres pdbqt text, in order of appearance."""
+    seen = []
+    for line in text.splitlines():
+        if line.startswith("ATOM"):
+            t = line[77:].split()[0]
+            if t not in seen:
+                seen.append(t)
+    return seen
+
+
 def write_ligand(lig: Ligand, path: str) -> str:
     with open(path, "w") as fh:
         fh.write(lig.pdbqt())

@@ -133,7 +133,8 @@ const void* ref_ptr(void* h, const char* name) {
 // (host/src/processresult.cpp:430-466).  Writes inter and intra energies,
 // and, if xyz != NULL, the atom coordinates in grid units after
 // change_conform_f, and per-atom vdW/elec inter energies if peratom != NULL
-// (2*num_atoms floats).
+// (2*num_atoms floats).  Inter includes the grid energy of flexible-residue
+// atoms.
 void ref_score(void* h, const float* genotype, double* inter, double* intra, double* xyz, float* peratom) {
     RefHandle* H = (RefHandle*)h;
     Liganddata lig = H->lig_ref;
@@ -149,7 +150,10 @@ void ref_score(void* h, const float* genotype, double* inter, double* intra, dou
         ie = calc_interE_f(&H->grid, &lig, 0.0005, 0, intraflexE);
     scale_ligand(&lig, H->grid.spacing);
     float ia = calc_intraE_f(&lig, 8, H->pars.smooth, 0, H->pars.elec_min_distance, H->tables, 0, interflexE);
-    *inter = ie;
+    // calc_interE_f books the grid energy of flexible-residue atoms in
+    // intraflexE; the GPU energy (calcenergy.cl:224-315) counts every atom as
+    // intermolecular, so both parts are returned together.
+    *inter = (double)ie + (double)intraflexE;
     *intra = ia;
 }
 

@@ -9,7 +9,7 @@ data files (run in a container where the reference tree is mounted):
   ref_<name>.npz -- device tables, initial population and PRNG seeds that
       the reference's own host code (oracle/_ref/libadref.so) derives for a
       synthetic ligand and map set (used by GPU tests on boxes without the
-      reference tree).
+      reference tree); ref_flex.npz adds two flexible residues (--flexres).
 
 Usage: python tests/golden/make_golden.py [/root/reference]
 """
@@ -134,8 +134,47 @@ def ref_fixture(name, n_atoms, n_tors, lig_seed, map_seed, npts, nrun, psize, se
     print("ref_%s.npz: %d atoms, %d torsions, %d pairs" % (name, R.num_atoms, R.num_rotbonds, R.num_pairs))
 
 
+def flex_fixture(name, lig_seed=7, map_seed=11, flex_seeds=(21, 22), npts=60, nrun=4, psize=16, seed=(5, 9)):
+    """Ligand of the 30-atom / 6-torsion case plus two flexible residues of
+    12 atoms and 4 torsions each: device tables as the reference's host code
+    builds them with --flexres (ligand atoms first, residue atoms after
+    them, residue torsions after the ligand's)."""
+    import oracle as O
+    from adgpu import synth
+    lig = synth.make_ligand(30, 6, seed=lig_seed)
+    offsets = ((7.0, 11.0, 11.0), (15.0, 11.0, 11.0))
+    flex = [synth.make_flexres(12, 4, seed=s, offset=o, resnum=50 + k)
+            for k, (s, o) in enumerate(zip(flex_seeds, offsets))]
+    types = list(lig.type_names)
+    for f in flex:
+        types += [t for t in synth.flexres_types(f) if t not in types]
+    maps = synth.make_maps(types, npts=npts, seed=map_seed)
+    d = tempfile.mkdtemp()
+    fld = synth.write_maps(maps, d)
+    lp = synth.write_ligand(lig, os.path.join(d, "lig.pdbqt"))
+    fp = os.path.join(d, "flex.pdbqt")
+    with open(fp, "w") as fh:
+        fh.write("".join(flex))
+    R = O.RefSetup(fld, lp, nrun=nrun, psize=psize, seed=seed, extra=["--flexres", fp])
+    T = R.tables()
+    out = {k: np.asarray(getattr(T, k)) for k in ("charges", "atom_types", "atom_map", "ignore_inter", "ref_coords",
+                                                   "rotbond_moving", "rotbond_unit", "rotbond_atoms", "atom_rotbonds",
+                                                   "pairs", "atype_reqm", "vw_exp", "reqm_ab", "vw_ac", "vw_bd",
+                                                   "ds_s", "ds_v")}
+    out.update(num_atoms=R.num_atoms, true_ligand_atoms=R.true_ligand_atoms, num_rotbonds=R.num_rotbonds,
+               num_atypes=R.num_atypes, init_pop=R.init_pop(), seeds=R.seeds(), nrun=nrun, psize=psize,
+               seed=np.array(seed), size=np.array(R.size), spacing=R.spacing, num_map_atypes=R.num_map_atypes,
+               size_angstr=np.array(R.size_angstr), coeff_elec=R.coeff_elec, coeff_desolv=R.coeff_desolv,
+               qasp=R.qasp, smooth=R.smooth, elec_min=R.elec_min, lig_seed=lig_seed, map_seed=map_seed,
+               npts=npts, map_types=np.array(types))
+    np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), **out)
+    print("ref_%s.npz: %d atoms (%d ligand), %d torsions, %d pairs, %d ignored" % (
+        name, R.num_atoms, R.true_ligand_atoms, R.num_rotbonds, R.num_pairs, int(np.sum(out["ignore_inter"]))))
+
+
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     for c in ("1stp", "3ce3"):
         dlg_fixture(ref, c)
     ref_fixture("c1", 30, 6, lig_seed=7, map_seed=11, npts=60, nrun=4, psize=16, seed=(5, 9))
+    flex_fixture("flex")

@@ -143,3 +143,24 @@ def angle_diff(a, b):
     """Difference of angles in degrees modulo 360, in [0, 180]."""
     d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) % 360.0
     return np.minimum(d, 360.0 - d)
+
+
+def fixture_case(name):
+    """Case built on a reference-made table fixture (tests/golden/ref_<name>.npz)
+    with the synthetic maps the fixture was generated against."""
+    f = load_ref_fixture(name)
+    T = tables_from_fixture(f)
+    types = [str(t) for t in f["map_types"]] if "map_types" in f else None
+    if types is None:
+        maps = synth.make_maps(synth.make_ligand(int(f["n_atoms_req"]), int(f["n_tors_req"]),
+                                                 seed=int(f["lig_seed"])).type_names,
+                               npts=int(f["npts"]), seed=int(f["map_seed"]))
+    else:
+        maps = synth.make_maps(types, npts=int(f["npts"]), seed=int(f["map_seed"]))
+    packed = maps.packed()
+    ep = energy_params(coeff_elec=float(f["coeff_elec"]), coeff_desolv=float(f["coeff_desolv"]),
+                       qasp=float(f["qasp"]), smooth=float(f["smooth"]), elec_min_distance=float(f["elec_min"]))
+    cg = CGrid(packed, maps.size, maps.num_map_atypes, maps.spacing)
+    cl = CLigand(T)
+    return SimpleNamespace(fixture=f, maps=maps, tables=T, packed=packed, ep=ep, cg=cg, cl=cl,
+                           orc=O.Oracle(cg, cl, ep), ng=T.num_genes, size=maps.size, spacing=maps.spacing)

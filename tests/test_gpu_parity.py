@@ -247,3 +247,92 @@ def test_job_best_f64(gpu_ctx):
         # the FP32 search energy of the same pose is within the FP32 bar of its FP64 value
         assert abs(b["energy_f32"][r] - e64) <= 1e-4 * max(1.0, s) + 1e-3
     job.destroy()
+
+
+@pytest.fixture(scope="module")
+def flex_case():
+    return helpers.fixture_case("flex")
+
+
+def test_flexres_energy_and_gradient_parity(gpu_ctx, flex_case):
+    """Ligand plus two flexible residues, tables made by the reference's host
+    code with --flexres (tests/golden/ref_flex.npz): residue atoms take only
+    their own torsions, the first two atoms of each residue have no grid
+    energy, ligand-residue pairs are intramolecular.  Energy bar and merged
+    gradient bar as for the ligand-only cases, FP32 and BF16 maps."""
+    case = flex_case
+    poses = np.concatenate([case.fixture["init_pop"], helpers.poses_for(case, 800, seed=707)])
+    mid, lid = _upload(gpu_ctx, case)
+    e, _ = gpu_ctx.eval_poses(mid, lid, poses, adgpu.EVAL_ENERGY)
+    helpers.check_energies(case.orc, poses, e)
+    e2, g = gpu_ctx.eval_poses(mid, lid, poses, adgpu.EVAL_ENERGRAD)
+    helpers.check_energies(case.orc, poses, e2, e_ref=e)
+    ng = case.ng
+    qv = np.array([helpers.gene_quantum(case, k) for k in range(ng)])
+    well = [0, 1, 2] + list(range(6, ng))
+    viol = []
+    for i, p in enumerate(poses):
+        _, go = case.orc.energrad32(p, True)
+        d = np.abs(g[i, :ng] - go)
+        tol = 1e-4 * np.abs(go) + 4 * qv
+        viol.extend([(i, k, float(g[i, k]), float(go[k])) for k in well if d[k] > tol[k]])
+    assert len(viol) <= 0.002 * len(poses) * len(well), viol[:10]
+    assert np.all(g[:, ng:] == 0.0)
+    # BF16 map storage against the oracle on the same rounded values
+    u = case.packed.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    rounded = u.astype(np.uint32).view(np.float32)
+    mid16 = gpu_ctx.maps_upload(case.packed, case.size, case.maps.num_map_atypes, case.spacing, adgpu.MAP_BF16)
+    e3, _ = gpu_ctx.eval_poses(mid16, lid, poses, adgpu.EVAL_ENERGY)
+    orc16 = O.Oracle(CGrid(rounded, case.size, case.maps.num_map_atypes, case.spacing), case.cl, case.ep)
+    helpers.check_energies(orc16, poses, e3)
+    gpu_ctx.maps_release(mid16)
+
+
+def test_flexres_float_gradient_and_fp64(gpu_ctx, flex_case):
+    """FIRE's float gradient (ligand atoms only in the translation and rotation
+    genes, calcgradient.cl) within 2x the FP32 oracle's own error against
+    FP64, and FP64 re-scoring equal to the FP64 oracle (1e-10 of the sum of
+    |terms|), with flexible residues."""
+    case = flex_case
+    mid, lid = _upload(gpu_ctx, case)
+    poses = helpers.poses_for(case, 300, seed=808)
+    _, g = gpu_ctx.eval_poses(mid, lid, poses, adgpu.EVAL_GRADIENT)
+    ng = case.ng
+    eg, e3, scale = [], [], []
+    for i, p in enumerate(poses):
+        g64 = case.orc.gradient64(p)
+        eg.append(np.abs(g[i, :ng] - g64))
+        e3.append(np.abs(case.orc.gradient32(p) - g64))
+        scale.append(np.abs(g64))
+    eg, e3, scale = np.array(eg), np.array(e3), np.array(scale)
+    for pct in (50, 99):
+        lhs = np.percentile(eg, pct, axis=0)
+        rhs = 2 * np.percentile(e3, pct, axis=0) + 1e-4 * np.percentile(scale, pct, axis=0)
+        assert np.all(lhs <= rhs), (pct, lhs, rhs)
+    tot, inter, intra = gpu_ctx.rescore_f64(mid, lid, poses)
+    for i, p in enumerate(poses):
+        e, ei, ea, s = case.orc.energy64(p)
+        tol = 1e-10 * max(1.0, s)
+        assert abs(tot[i] - e) <= tol and abs(inter[i] - ei) <= tol and abs(intra[i] - ea) <= tol, i
+
+
+def test_flexres_lga_job(gpu_ctx, flex_case):
+    """A short LGA + ADADELTA job on the flexible-residue case from the
+    reference's own initial population and seeds: the downloaded energies are
+    the oracle's FP32 energies of the downloaded genotypes, and the search
+    lowers every run's best energy below its initial best."""
+    case = flex_case
+    f = case.fixture
+    mid, lid = _upload(gpu_ctx, case)
+    nrun, psize = int(f["nrun"]), int(f["psize"])
+    sp = search_params(num_runs=nrun, pop_size=psize, max_evals=30000, ls="ad", spacing=case.spacing)
+    job = gpu_ctx.job(mid, lid, sp, f["init_pop"], f["seeds"])
+    job.initpop()
+    e0 = job.download()["ene"].reshape(nrun, psize).min(axis=1)
+    job.run()
+    st = job.download()
+    job.destroy()
+    pop = st["pop"].reshape(nrun * psize, -1)
+    helpers.check_energies(case.orc, pop, st["ene"].reshape(-1))
+    assert np.all(st["ene"].reshape(nrun, psize).min(axis=1) < e0)

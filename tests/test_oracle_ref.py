@@ -99,3 +99,52 @@ def test_coords_and_inter_energy_vs_reference_cpu_path():
         # calc_interE_f works in float32; the rigid re-placement rounds the
         # coordinates to float32 as well
         assert e_inter == pytest.approx(rs["inter"], rel=1e-4, abs=1e-3)
+
+
+def test_flexible_residues_vs_reference_cpu_path():
+    """A ligand plus two flexible residues given with --flexres: residue atoms
+    follow the ligand atoms, only ligand atoms take the general rotation and
+    translation, the first two atoms of each residue are excluded from the
+    intermolecular energy (processligand.cpp:1245-1250).  The oracle's
+    coordinates equal change_conform_f and its intermolecular energy equals
+    calc_interE_f; the committed fixture tables equal a fresh reference build."""
+    lig = synth.make_ligand(30, 6, seed=7)
+    offsets = ((7.0, 11.0, 11.0), (15.0, 11.0, 11.0))
+    flex = [synth.make_flexres(12, 4, seed=s, offset=o, resnum=50 + k) for k, (s, o) in enumerate(zip((21, 22), offsets))]
+    types = list(lig.type_names)
+    for f in flex:
+        types += [t for t in synth.flexres_types(f) if t not in types]
+    maps = synth.make_maps(types, npts=60, seed=11)
+    d = tempfile.mkdtemp()
+    fld = synth.write_maps(maps, d)
+    lp = synth.write_ligand(lig, os.path.join(d, "lig.pdbqt"))
+    fp = os.path.join(d, "flex.pdbqt")
+    with open(fp, "w") as fh:
+        fh.write("".join(flex))
+    R = O.RefSetup(fld, lp, nrun=4, psize=16, seed=(5, 9), extra=["--flexres", fp])
+    RT = R.tables()
+    assert (R.num_atoms, R.true_ligand_atoms, R.num_rotbonds) == (54, 30, 14)
+    assert list(np.nonzero(RT.ignore_inter)[0]) == [30, 31, 42, 43]
+    fx = helpers.load_ref_fixture("flex")
+    for k in FIELDS + ("ignore_inter",):
+        assert np.array_equal(np.asarray(getattr(RT, k)), fx[k]), k
+    ep = energy_params(coeff_elec=R.coeff_elec, coeff_desolv=R.coeff_desolv, qasp=R.qasp, smooth=R.smooth,
+                       elec_min_distance=R.elec_min)
+    cg = CGrid(R.grids(), R.size, R.num_map_atypes, R.spacing)
+    orc = O.Oracle(cg, CLigand(RT), ep)
+    rng = np.random.default_rng(3)
+    poses = R.init_pop()[:40].copy()
+    poses[:, 6:20] = rng.uniform(0, 360, size=(40, 14)).astype(np.float32)
+    lever = np.max(np.abs(orc.coords64(np.zeros(64, np.float32)))) + 1.0
+    for g in poses:
+        rs = R.score(g)
+        xyz = orc.coords64(g)
+        assert np.max(np.abs(xyz - rs["xyz"])) < 2e-6 * 360 * lever
+        # the reference's coordinates through the oracle's rigid evaluation,
+        # with the residue root atoms excluded as in the reference
+        t, centre = helpers.rigid_tables(RT.atom_map, RT.charges, rs["xyz"])
+        t.ignore_inter = np.asarray(RT.ignore_inter, np.uint8)
+        gg = np.zeros(64, np.float32)
+        gg[0:3] = centre
+        e_inter = O.Oracle(cg, CLigand(t), ep).energy64(gg)[1]
+        assert e_inter == pytest.approx(rs["inter"], rel=1e-4, abs=1e-3)
