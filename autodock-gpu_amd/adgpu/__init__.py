@@ -209,3 +209,25 @@ class Job:
         ms, n = C.c_double(), C.c_int64()
         self._c(self.lib.adgpu_job_kernel_time(self.h, cls, C.byref(ms), C.byref(n)), "adgpu_job_kernel_time")
         return ms.value, n.value
+
+    def set_autostop(self, enable=True, stopstd=0.15, as_frequency=5, min_as_evals=0):
+        """AutoStop tested on the device every as_frequency generations."""
+        self._c(self.lib.adgpu_job_set_autostop(self.h, int(bool(enable)), float(stopstd), int(as_frequency),
+                                                int(min_as_evals)), "adgpu_job_set_autostop")
+
+    def autostop_check(self):
+        r = _abi.AutostopRecord()
+        self._c(self.lib.adgpu_job_autostop_check(self.h, C.byref(r)), "adgpu_job_autostop_check")
+        return r.as_dict()
+
+    def autostop_log(self):
+        n = C.c_int()
+        self._c(self.lib.adgpu_job_autostop_log(self.h, 0, 0, None, C.byref(n)), "adgpu_job_autostop_log")
+        recs = (_abi.AutostopRecord * max(1, n.value))()
+        self._c(self.lib.adgpu_job_autostop_log(self.h, 0, n.value, recs, C.byref(n)), "adgpu_job_autostop_log")
+        return [recs[i].as_dict() for i in range(n.value)]
+
+    def autostop_final(self):
+        r = _abi.AutostopRecord()
+        self._c(self.lib.adgpu_job_autostop_final(self.h, C.byref(r)), "adgpu_job_autostop_final")
+        return r.as_dict()

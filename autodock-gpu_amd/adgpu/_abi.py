@@ -60,6 +60,16 @@ class SearchParams(C.Structure):
                 ("adam_beta1", C.c_float), ("adam_beta2", C.c_float), ("adam_epsilon", C.c_float)]
 
 
+class AutostopRecord(C.Structure):
+    _fields_ = [("generation", C.c_int64), ("total_evals", C.c_int64), ("threshold", C.c_float),
+                ("average", C.c_float), ("stddev", C.c_float), ("samples", C.c_int32), ("best_energy", C.c_float),
+                ("autostopped", C.c_int32), ("stop", C.c_int32), ("roll_average", C.c_float),
+                ("roll_stddev", C.c_float), ("roll_samples", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _ptr(a, ctype):
     return a.ctypes.data_as(ctype)
 
@@ -178,6 +188,11 @@ def load_library(path: str | None = None):
     lib.adgpu_job_download.argtypes = [vp, _fp, _fp, _ip, _up]
     lib.adgpu_job_upload.argtypes = [vp, _fp, _fp, _up]
     lib.adgpu_job_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    rp = C.POINTER(AutostopRecord)
+    lib.adgpu_job_set_autostop.argtypes = [vp, C.c_int, C.c_float, C.c_int, C.c_int64]
+    lib.adgpu_job_autostop_check.argtypes = [vp, rp]
+    lib.adgpu_job_autostop_log.argtypes = [vp, C.c_int, C.c_int, rp, C.POINTER(C.c_int)]
+    lib.adgpu_job_autostop_final.argtypes = [vp, rp]
     if path is None:
         _LIB = lib
     return lib
@@ -187,4 +202,5 @@ EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "a
             "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses",
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",
             "adgpu_job_kernel_ls", "adgpu_job_kernel_sum_and_swap", "adgpu_job_generation", "adgpu_job_run",
-            "adgpu_job_download", "adgpu_job_upload", "adgpu_job_kernel_time"]
+            "adgpu_job_download", "adgpu_job_upload", "adgpu_job_kernel_time", "adgpu_job_set_autostop",
+            "adgpu_job_autostop_check", "adgpu_job_autostop_log", "adgpu_job_autostop_final"]

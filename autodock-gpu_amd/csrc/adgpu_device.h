@@ -67,7 +67,6 @@ struct Team {
     static_assert(TW == 32 || TW == 64, "team width");
     static constexpr int kPerWave = 64 / TW;
     static constexpr int kGPL = 64 / TW;          // genes per lane
-    static constexpr int kPerGroup = 256 / TW;    // teams per 256-thread workgroup
     DEV static int lane() { return lane_id() & (TW - 1); }
     DEV static int slot() { return lane_id() / TW; }
     // Sum over the team; every lane of the team receives the same bits.
@@ -105,6 +104,12 @@ struct Team {
 DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 DEV float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 DEV float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// LDS atomic subtract without a returned value (ds_sub_u32; HIP's atomicSub
+// would negate and add).
+DEV void lds_sub(int* p, int v) {
+    (void)__hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // fmod_pi2, calcenergy.cl:86-91
 DEV float fmod_pi2(float x) { return x - (float)((int)((1.0f / kTwoPi) * x)) * kTwoPi; }
@@ -225,8 +230,9 @@ __host__ __device__ inline size_t team_bytes_for(int natoms_pad, int T) {
     return 6 * align16((size_t)natoms_pad * 4) + (size_t)T * 48 + align16((size_t)T * 4) + 48;
 }
 
-__host__ __device__ inline size_t lds_bytes_for(int TW, int NC, int T, int P, bool pairs_in_lds) {
-    return shared_bytes_for(T, P, pairs_in_lds) + (size_t)(256 / TW) * team_bytes_for(TW * NC, T);
+// LDS of one workgroup of `threads` threads (threads / TW teams).
+__host__ __device__ inline size_t lds_bytes_for(int TW, int NC, int T, int P, bool pairs_in_lds, int threads = 256) {
+    return shared_bytes_for(T, P, pairs_in_lds) + (size_t)(threads / TW) * team_bytes_for(TW * NC, T);
 }
 
 // Stages the shared tables (all threads of the workgroup) and carves the
@@ -595,10 +601,11 @@ DEV float evaluate(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float
         float dedr;
         e += pair_energy<GRAD>(X.ls, pa, pb, dist * spacing, rdist, G.inv_spacing, dedr);
         if (GRAD) {
-            const float gdd = -dedr * rdist;
-            const int ix = quant(kTermScale * sx * gdd), iy = quant(kTermScale * sy * gdd),
-                      iz = quant(kTermScale * sz * gdd);
-            atomicSub(&tv.fx[i], ix); atomicSub(&tv.fy[i], iy); atomicSub(&tv.fz[i], iz);
+            // TERMSCALE * s * gdd with the power-of-two scale folded into gdd:
+            // the same rounded product
+            const float gdd = kTermScale * (-dedr * rdist);
+            const int ix = quant(sx * gdd), iy = quant(sy * gdd), iz = quant(sz * gdd);
+            lds_sub(&tv.fx[i], ix); lds_sub(&tv.fy[i], iy); lds_sub(&tv.fz[i], iz);
             atomicAdd(&tv.fx[j], ix); atomicAdd(&tv.fy[j], iy); atomicAdd(&tv.fz[j], iz);
         }
     }
@@ -908,6 +915,9 @@ struct Launch {
     float gene_scale;    // SW: 1/sqrt(genes)
 };
 
+// A passed AutoStop test has ended the job: generation kernels do nothing.
+#define JOB_STOPPED(b) ((b).stop != nullptr && *(b).stop != 0)
+
 #define KERNEL_PROLOGUE(TW, NC, PL)                                            \
     extern __shared__ __align__(16) char lds[];                                \
     using TM = Team<TW>;                                                       \
@@ -920,7 +930,7 @@ struct Launch {
     X.ls.load(a.L);                                                            \
     const int tl = TM::lane();                                                 \
     (void)tl;                                                                  \
-    const int team_idx = blockIdx.x * TM::kPerGroup + (int)(threadIdx.x / TW); \
+    const int team_idx = (int)(blockIdx.x * (blockDim.x / TW) + threadIdx.x / TW); \
     (void)team_idx;
 
 // Entity choice of every LS kernel (e.g. kernel3.cl:104-117): team lane 0
@@ -946,22 +956,24 @@ struct Shape {
     int tw, nc;
     bool pl;
     size_t lds;
+    int threads;   // workgroup size (64, 128 or 256)
 };
 
 constexpr size_t kLdsBudget = 64 * 1024;
 
 // Chooses the team width and atom chunks for a ligand: 32-lane teams (two
 // individuals per wavefront) up to 128 atoms, 64-lane teams beyond.
-inline Shape choose_shape(const LigandDev& L, int team_width_hint) {
+inline Shape choose_shape(const LigandDev& L, int team_width_hint, int threads = 256) {
     Shape s{};
+    s.threads = threads;
     const int N = L.num_atoms;
     int tw = team_width_hint == 64 ? 64 : (team_width_hint == 32 ? 32 : (N <= 128 ? 32 : 64));
     if (tw == 32 && N > 128) tw = 64;
     s.tw = tw;
     const int per = (N + tw - 1) / tw;
     s.nc = per <= 1 ? 1 : (per <= 2 ? 2 : 4);
-    s.pl = lds_bytes_for(s.tw, s.nc, L.num_rotbonds, L.num_pairs, true) <= kLdsBudget;
-    s.lds = lds_bytes_for(s.tw, s.nc, L.num_rotbonds, L.num_pairs, s.pl);
+    s.pl = lds_bytes_for(s.tw, s.nc, L.num_rotbonds, L.num_pairs, true, threads) <= kLdsBudget;
+    s.lds = lds_bytes_for(s.tw, s.nc, L.num_rotbonds, L.num_pairs, s.pl, threads);
     return s;
 }
 

@@ -65,6 +65,7 @@ struct RunGroup {
     int r0 = 0, r1 = 0;                 // runs [r0, r1)
     hipStream_t stream = nullptr;
     int parity = 0;                     // current population in buffer A (0) or B (1)
+    int parity0 = 0;                    // parity when the current adgpu_job_run started
     int64_t done = 0, enq = 0;          // generations completed / enqueued in this adgpu_job_run
     std::vector<int64_t> totals;        // evals of the group's runs after each completed generation
     bool ls_used[kRing] = {};
@@ -89,6 +90,17 @@ struct adgpu_job {
     int64_t generation = 0;
     double kernel_ms[2] = {0.0, 0.0};
     int64_t kernel_n[2] = {0, 0};
+    // AutoStop on the device (k_autostop.hip)
+    int32_t* d_stop = nullptr;          // raised by a passed test; every generation kernel then returns
+    int32_t* h_stop = nullptr;          // pinned copy, written after each test
+    bool stopped = false;               // a passed test has ended the job
+    bool as_enabled = false;
+    AutoStopParams as_p{};
+    AutoStopDev* d_as = nullptr;
+    float* d_as_bins = nullptr;
+    adgpu_autostop_record* d_as_log = nullptr;
+    adgpu_autostop_record* d_as_out = nullptr;
+    hipEvent_t ev_test = nullptr;
 };
 
 namespace {
@@ -536,6 +548,9 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
               hipMalloc(&j->evals_runs, nrun * sizeof(int32_t)) == hipSuccess &&
               hipMalloc(&j->prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&j->counters, 16 * ng * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&j->d_stop, 16 * sizeof(int32_t)) == hipSuccess &&
+              hipHostMalloc(&j->h_stop, 16 * sizeof(int32_t)) == hipSuccess &&
+              hipEventCreateWithFlags(&j->ev_test, hipEventDisableTiming) == hipSuccess &&
               hipHostMalloc(&j->h_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess;
     if (ok && sp->ls_method == ADGPU_LS_FIRE) {
         const size_t nls = (size_t)nrun * (j->sd.num_ls > 0 ? j->sd.num_ls : 1);
@@ -565,6 +580,8 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     HIPCHK(c, hipMemset(j->ene[0], 0, ne * sizeof(float)));
     HIPCHK(c, hipMemset(j->ene[1], 0, ne * sizeof(float)));
     HIPCHK(c, hipMemset(j->counters, 0, 16 * ng * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(j->d_stop, 0, 16 * sizeof(int32_t)));
+    j->h_stop[0] = 0;
     *out = j.release();
     return ADGPU_OK;
 }
@@ -586,6 +603,10 @@ int adgpu_job_destroy(adgpu_job_t j) {
     (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
     (void)hipFree(j->evals_new); (void)hipFree(j->evals_runs); (void)hipFree(j->prng);
     (void)hipFree(j->counters); (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
+    (void)hipFree(j->d_stop); (void)hipFree(j->d_as); (void)hipFree(j->d_as_bins);
+    (void)hipFree(j->d_as_log); (void)hipFree(j->d_as_out);
+    if (j->ev_test) (void)hipEventDestroy(j->ev_test);
+    if (j->h_stop) (void)hipHostFree(j->h_stop);
     if (j->h_ring) (void)hipHostFree(j->h_ring);
     delete j;
     return ADGPU_OK;
@@ -616,6 +637,7 @@ static JobBuffers slice(const adgpu_job* j, int r0, int r1, int parity, int coun
     const size_t nls = (size_t)(j->sd.num_ls > 0 ? j->sd.num_ls : 1);
     b.ls_ent = j->ls_ent ? j->ls_ent + (size_t)r0 * nls : nullptr;
     b.ls_grad = j->ls_grad ? j->ls_grad + (size_t)r0 * nls * kGenes : nullptr;
+    b.stop = j->d_stop;
     (void)r1;
     return b;
 }
@@ -770,6 +792,37 @@ int adgpu_job_run(adgpu_job_t j, int64_t* out_gens, int64_t* out_total) {
     return adgpu_job_run_generations(j, INT64_MAX, out_gens, out_total);
 }
 
+// Enqueues the AutoStop test after cg generations of this call on the first
+// group's stream, behind every group's generation cg-1; the groups' later
+// generations wait for it (performdocking.cpp.Cuda:687-696).
+static int enqueue_test(adgpu_job_t j, int64_t cg) {
+    adgpu_context* c = j->ctx;
+    hipStream_t s0 = j->groups[0].stream;
+    if (cg > 0)
+        for (size_t k = 1; k < j->groups.size(); ++k)
+            HIPCHK(c, hipStreamWaitEvent(s0, j->groups[k].ev_done[(cg - 1) % kRing], 0));
+    const float* ene = j->ene[j->groups[0].parity0 ^ (int)(cg & 1)];
+    HIPCHK(c, launch_autostop(s0, j->as_p, ene, j->evals_runs, j->d_as, j->d_as_bins, j->d_as_log, j->d_stop,
+                              j->generation + cg, 0, nullptr));
+    HIPCHK(c, hipMemcpyAsync(j->h_stop, j->d_stop, sizeof(int32_t), hipMemcpyDeviceToHost, s0));
+    HIPCHK(c, hipEventRecord(j->ev_test, s0));
+    for (size_t k = 1; k < j->groups.size(); ++k) HIPCHK(c, hipStreamWaitEvent(j->groups[k].stream, j->ev_test, 0));
+    return ADGPU_OK;
+}
+
+// Generation count of the test that raised the stop flag (the last test run).
+static int stopped_generation(adgpu_job_t j, int64_t* gen) {
+    adgpu_context* c = j->ctx;
+    AutoStopDev st{};
+    HIPCHK(c, hipStreamSynchronize(j->groups[0].stream));
+    HIPCHK(c, hipMemcpy(&st, j->d_as, sizeof(st), hipMemcpyDeviceToHost));
+    if (st.num_tests < 1) return fail(c, ADGPU_ERR_STATE, "stop flag without an AutoStop test");
+    adgpu_autostop_record r{};
+    HIPCHK(c, hipMemcpy(&r, j->d_as_log + (st.num_tests - 1) % j->as_p.log_cap, sizeof(r), hipMemcpyDeviceToHost));
+    *gen = r.generation;
+    return ADGPU_OK;
+}
+
 int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens, int64_t* out_total) {
     if (!j || max_new < 0) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
@@ -785,7 +838,9 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
         for (int r = g.r0; r < g.r1; ++r) t += j->h_ring[r];
         g.totals.assign(1, t);
         g.done = g.enq = 0;
+        g.parity0 = g.parity;
     }
+    if (j->stopped) max_new = 0;   // a passed AutoStop test ended the job
     // most evals one run can add in one generation: K4 offspring plus every LS
     // slot at its iteration cap (two evaluations per Solis-Wets iteration)
     const int64_t emax = (int64_t)(j->sp.pop_size - 1) +
@@ -801,29 +856,188 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
     auto surely_continues = [&](int64_t cg) {
         return cg < max_new && progress_of(j, bound(cg), j->generation + cg) < 100.0;
     };
-    for (;;) {
-        for (int k = 0; k < (int)j->groups.size(); ++k) {
-            RunGroup& g = j->groups[k];
-            while (g.enq - g.done < kRing && surely_continues(g.enq))
-                if ((rc = enqueue_generation(j, k, ls, ms))) return rc;
+    // AutoStop: the test after cg generations (generation count j->generation
+    // + cg, a multiple of as_frequency) precedes generation cg + 1 of every
+    // group; next_test is the first test point not enqueued yet
+    const int64_t asf = j->as_enabled ? j->as_p.as_frequency : 0;
+    int64_t next_test = asf ? (asf - j->generation % asf) % asf : INT64_MAX;
+    bool test_pending = false, stop_known = false;
+    int64_t stop_cg = -1;
+    auto poll_test = [&]() -> int {
+        if (!test_pending) return ADGPU_OK;
+        const hipError_t q = hipEventQuery(j->ev_test);
+        if (q == hipErrorNotReady) return ADGPU_OK;
+        HIPCHK(c, q);
+        test_pending = false;
+        if (j->h_stop[0] && !stop_known) {
+            int64_t gen = 0;
+            const int r = stopped_generation(j, &gen);
+            if (r) return r;
+            stop_known = true;
+            stop_cg = gen - j->generation;
         }
+        return ADGPU_OK;
+    };
+    for (;;) {
+        for (bool moved = true; moved && !stop_known;) {
+            moved = false;
+            bool all_at_test = asf != 0;
+            for (auto& g : j->groups) all_at_test = all_at_test && g.enq >= next_test;
+            if (all_at_test && surely_continues(next_test)) {
+                if ((rc = enqueue_test(j, next_test))) return rc;
+                test_pending = true;
+                next_test += asf;
+                moved = true;
+            }
+            for (int k = 0; k < (int)j->groups.size(); ++k) {
+                RunGroup& g = j->groups[k];
+                while (g.enq - g.done < kRing && g.enq < next_test && surely_continues(g.enq)) {
+                    if ((rc = enqueue_generation(j, k, ls, ms))) return rc;
+                    moved = true;
+                }
+            }
+        }
+        if ((rc = poll_test())) return rc;
         int wait_k = -1;
         for (int k = 0; k < (int)j->groups.size(); ++k) {
             const RunGroup& g = j->groups[k];
             if (g.enq > g.done && (wait_k < 0 || g.done < j->groups[wait_k].done)) wait_k = k;
         }
-        if (wait_k < 0) break;   // nothing in flight and nothing may be enqueued: all stopped at the final generation
+        if (wait_k < 0) {
+            if (!test_pending) break;   // nothing in flight and nothing may be enqueued
+            HIPCHK(c, hipEventSynchronize(j->ev_test));
+            continue;
+        }
         if ((rc = complete_generation(j, wait_k))) return rc;
     }
-    const int64_t gens = j->groups[0].done;
+    int64_t gens = j->groups[0].done;
+    if (stop_known) {
+        // generations enqueued behind the passed test did nothing
+        gens = stop_cg;
+        for (auto& g : j->groups) {
+            if (g.done < gens) return fail(c, ADGPU_ERR_STATE, "run group behind the AutoStop test");
+            g.parity = g.parity0 ^ (int)(gens & 1);
+        }
+        j->stopped = true;
+    }
     int64_t total = 0;
     for (auto& g : j->groups) {
-        if (g.done != gens) return fail(c, ADGPU_ERR_STATE, "run groups stopped at different generations");
+        if (!stop_known && g.done != gens) return fail(c, ADGPU_ERR_STATE, "run groups stopped at different generations");
         total += g.totals[gens];
     }
     j->generation += gens;
     if (out_gens) *out_gens = j->generation;
     if (out_total) *out_total = total;
+    return ADGPU_OK;
+}
+
+int adgpu_job_set_autostop(adgpu_job_t j, int enable, float stopstd, int as_frequency, int64_t min_as_evals) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    if (enable && (as_frequency < 1 || !(stopstd > 0.0f) || min_as_evals < 0))
+        return fail(c, ADGPU_ERR_ARG, "AutoStop needs as_frequency >= 1, stopstd > 0 and min_as_evals >= 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    const int pop = j->sp.pop_size;
+    AutoStopParams p{};
+    p.num_runs = j->sp.num_runs;
+    p.pop_size = pop;
+    p.ncream = pop / 10;
+    p.as_frequency = as_frequency;
+    p.stopstd = stopstd;
+    p.min_as_evals = min_as_evals;
+    // one record per test of the longest allowed search, at most 16384 kept
+    const int64_t tests = as_frequency > 0 ? j->sp.max_generations / as_frequency + 2 : 2;
+    p.log_cap = (int32_t)std::max<int64_t>(16, std::min<int64_t>(tests, 16384));
+    const size_t nb = 3 * ((size_t)pop + 1);
+    if (!j->d_as) {
+        if (hipMalloc(&j->d_as, sizeof(AutoStopDev)) != hipSuccess ||
+            hipMalloc(&j->d_as_bins, nb * sizeof(float)) != hipSuccess ||
+            hipMalloc(&j->d_as_out, sizeof(adgpu_autostop_record)) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "AutoStop allocation failed");
+    }
+    if (j->d_as_log && j->as_p.log_cap != p.log_cap) { (void)hipFree(j->d_as_log); j->d_as_log = nullptr; }
+    if (!j->d_as_log && hipMalloc(&j->d_as_log, (size_t)p.log_cap * sizeof(adgpu_autostop_record)) != hipSuccess)
+        return fail(c, ADGPU_ERR_NOMEM, "AutoStop allocation failed");
+    // the AutoStop constructor, autostop.hpp:129-146
+    AutoStopDev st{};
+    st.first_time = 1;
+    st.autostopped = 0;
+    st.threshold = (float)(1 << 24);
+    st.threshold_used = st.threshold;
+    st.thres_stddev = st.threshold;
+    st.curr_avg = (float)(-(1 << 24));
+    st.curr_std = st.thres_stddev;
+    st.roll_count = 0;
+    st.best_n = 1;
+    st.delta_energy = (float)(2.0 * (double)st.thres_stddev / (double)(uint32_t)pop);
+    st.overall_best = (float)(1 << 24);
+    st.num_tests = 0;
+    HIPCHK(c, hipMemcpy(j->d_as, &st, sizeof(st), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(j->d_as_bins, 0, nb * sizeof(float)));
+    HIPCHK(c, hipMemset(j->d_as_log, 0, (size_t)p.log_cap * sizeof(adgpu_autostop_record)));
+    HIPCHK(c, hipMemset(j->d_stop, 0, 16 * sizeof(int32_t)));
+    j->h_stop[0] = 0;
+    j->stopped = false;
+    j->as_p = p;
+    j->as_enabled = enable != 0;
+    return ADGPU_OK;
+}
+
+int adgpu_job_autostop_check(adgpu_job_t j, adgpu_autostop_record* out) {
+    if (!j) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    if (!j->d_as) return fail(c, ADGPU_ERR_STATE, "AutoStop not set up (adgpu_job_set_autostop)");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    hipStream_t s0 = j->groups[0].stream;
+    HIPCHK(c, launch_autostop(s0, j->as_p, j->ene[whole_parity(j)], j->evals_runs, j->d_as, j->d_as_bins,
+                              j->d_as_log, j->d_stop, j->generation, 0, nullptr));
+    HIPCHK(c, hipStreamSynchronize(s0));
+    AutoStopDev st{};
+    HIPCHK(c, hipMemcpy(&st, j->d_as, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(j->h_stop, j->d_stop, sizeof(int32_t), hipMemcpyDeviceToHost));
+    j->stopped = j->h_stop[0] != 0;
+    if (out) {
+        if (st.num_tests < 1) return fail(c, ADGPU_ERR_STATE, "no AutoStop test recorded");
+        HIPCHK(c, hipMemcpy(out, j->d_as_log + (st.num_tests - 1) % j->as_p.log_cap, sizeof(*out),
+                            hipMemcpyDeviceToHost));
+    }
+    return ADGPU_OK;
+}
+
+int adgpu_job_autostop_log(adgpu_job_t j, int first, int max, adgpu_autostop_record* out, int* out_count) {
+    if (!j || first < 0 || max < 0 || (max > 0 && !out)) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    if (!j->d_as) {
+        if (out_count) *out_count = 0;
+        return max > 0 ? fail(c, ADGPU_ERR_STATE, "AutoStop not set up (adgpu_job_set_autostop)") : ADGPU_OK;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    AutoStopDev st{};
+    HIPCHK(c, hipMemcpy(&st, j->d_as, sizeof(st), hipMemcpyDeviceToHost));
+    const int n = st.num_tests, cap = j->as_p.log_cap;
+    if (out_count) *out_count = n;
+    const int last = std::min(n, first + max);
+    if (first < last && first < n - cap)
+        return fail(c, ADGPU_ERR_ARG, "AutoStop records before the last log_cap tests are no longer kept");
+    for (int i = first; i < last; ++i)
+        HIPCHK(c, hipMemcpy(out + (i - first), j->d_as_log + i % cap, sizeof(*out), hipMemcpyDeviceToHost));
+    return ADGPU_OK;
+}
+
+int adgpu_job_autostop_final(adgpu_job_t j, adgpu_autostop_record* out) {
+    if (!j || !out) return ADGPU_ERR_ARG;
+    adgpu_context* c = j->ctx;
+    if (!j->d_as) return fail(c, ADGPU_ERR_STATE, "AutoStop not set up (adgpu_job_set_autostop)");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto& g : j->groups) HIPCHK(c, hipStreamSynchronize(g.stream));
+    hipStream_t s0 = j->groups[0].stream;
+    HIPCHK(c, launch_autostop(s0, j->as_p, j->ene[whole_parity(j)], j->evals_runs, j->d_as, j->d_as_bins,
+                              j->d_as_log, j->d_stop, j->generation, 1, j->d_as_out));
+    HIPCHK(c, hipStreamSynchronize(s0));
+    HIPCHK(c, hipMemcpy(out, j->d_as_out, sizeof(*out), hipMemcpyDeviceToHost));
     return ADGPU_OK;
 }
 

@@ -78,6 +78,24 @@ struct JobBuffers {
     int32_t* ls_counter;   // work queue of the persistent local-search kernels
     int32_t* ls_ent;       // FIRE: entity chosen per LS slot [runs*num_ls]
     float* ls_grad;        // FIRE: initial gradient per LS slot [runs*num_ls*64]
+    const int32_t* stop;   // set by a passed AutoStop test: generation kernels return at once
+};
+
+// The AutoStop class's members (host/inc/autostop.hpp:36-58) in device
+// memory; bins = average_sd2_N, 3*(pop_size+1) floats, follows the struct.
+struct AutoStopDev {
+    float threshold, threshold_used, thres_stddev, curr_avg, curr_std;
+    float delta_energy, overall_best;
+    float rolling[16];
+    uint32_t roll_count, best_n;
+    int32_t first_time, autostopped, num_tests;
+};
+
+struct AutoStopParams {
+    int32_t num_runs, pop_size, ncream, as_frequency;
+    float stopstd;
+    int32_t log_cap;           // records kept (the latest log_cap tests)
+    int64_t min_as_evals;
 };
 
 // What every launcher needs about the ligand and the maps.
@@ -106,6 +124,11 @@ hipError_t launch_ls_sw(const LaunchArgs& la, const SearchDev& sp, const JobBuff
 hipError_t launch_ls_ad(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
 hipError_t launch_ls_adam(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
 hipError_t launch_ls_fire(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b);
+// One AutoStop test (mode 0) or the final statistics (mode 1) on the
+// energies ene[num_runs*pop_size]; k_autostop.hip.
+hipError_t launch_autostop(hipStream_t s, const AutoStopParams& p, const float* ene, const int32_t* evals_runs,
+                           AutoStopDev* st, float* bins, adgpu_autostop_record* log, int32_t* stop,
+                           int64_t generation, int mode, adgpu_autostop_record* out);
 // LDS bytes and team width the launchers use for a ligand.
 size_t lds_bytes(const LigandDev& lig_host, int team_hint, int* team_width);
 

@@ -72,6 +72,7 @@ DEV void elite_of_run(const float* ecur, int pop, float& best_e, int& best_id) {
 // copied energy and its LCG streams unchanged as the reference does.
 template <int TW, int NC, int MF, bool PL>
 __global__ ADGPU_GROUP_KERNEL void k_gen(Launch a) {
+    if (JOB_STOPPED(a.B)) return;
     KERNEL_PROLOGUE(TW, NC, PL)
     const int pop = a.S.pop_size, NG = a.S.num_genes;
     const int ne = a.S.num_runs * pop;
@@ -150,6 +151,7 @@ __global__ ADGPU_GROUP_KERNEL void k_gen(Launch a) {
 
 // K2: per-run eval totals (kernel2.cl:26-62); one wavefront per run.
 __global__ __launch_bounds__(64) void k_sum_evals(SearchDev S, JobBuffers B) {
+    if (JOB_STOPPED(B)) return;
     const int run = blockIdx.x;
     const int lane = lane_id();
     int s = 0;

@@ -13,6 +13,7 @@ namespace adgpu {
 // gradient of its genotype, kept in ls_ent / ls_grad for k_ls_fire.
 template <int TW, int NC, int MF, bool PL>
 __global__ ADGPU_GROUP_KERNEL void k_fire_init(Launch a) {
+    if (JOB_STOPPED(a.B)) return;
     KERNEL_PROLOGUE(TW, NC, PL)
     const SearchDev& S = a.S;
     const int nls = S.num_ls, pop = S.pop_size, NG = S.num_genes;
@@ -45,6 +46,7 @@ __global__ ADGPU_GROUP_KERNEL void k_fire_init(Launch a) {
 // dt <= 1e-6.
 template <int TW, int NC, int MF, bool PL>
 __global__ ADGPU_GROUP_KERNEL void k_ls_fire(Launch a) {
+    if (JOB_STOPPED(a.B)) return;
     KERNEL_PROLOGUE(TW, NC, PL)
     const SearchDev& S = a.S;
     const int nls = S.num_ls, pop = S.pop_size, NG = S.num_genes;
@@ -156,10 +158,13 @@ hipError_t launch_ls_fire(const LaunchArgs& la, const SearchDev& sp, const JobBu
         err = hipGetLastError();
     });
     if (err != hipSuccess) return err;
-    ADGPU_DISPATCH(k_ls_fire, sh, la.grid.format, {
-        const int cap = resident_groups(reinterpret_cast<const void*>(kfn), sh.lds) / (la.share > 1 ? la.share : 1);
-        const int groups = cap > 0 && cap < want ? cap : want;
-        hipLaunchKernelGGL(kfn, dim3(groups), dim3(256), sh.lds, la.stream, a);
+    // the persistent loop, in workgroups of ls_group_threads()
+    const Shape shl = choose_shape(*la.lig_host, la.team_hint, ls_group_threads());
+    const int want_l = (n + shl.threads / shl.tw - 1) / (shl.threads / shl.tw);
+    ADGPU_DISPATCH(k_ls_fire, shl, la.grid.format, {
+        const int cap = persistent_cap(reinterpret_cast<const void*>(kfn), shl.lds, shl.threads, la.share);
+        const int groups = cap > 0 && cap < want_l ? cap : want_l;
+        hipLaunchKernelGGL(kfn, dim3(groups), dim3(shl.threads), shl.lds, la.stream, a);
         err = hipGetLastError();
     });
     return err;

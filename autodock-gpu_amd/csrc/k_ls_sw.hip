@@ -19,6 +19,7 @@ namespace adgpu {
 // Gene g draws from its own LCG stream (work-item g of the reference).
 template <int TW, int NC, int MF, bool PL>
 __global__ ADGPU_GROUP_KERNEL void k_ls_sw(Launch a) {
+    if (JOB_STOPPED(a.B)) return;
     KERNEL_PROLOGUE(TW, NC, PL)
     const SearchDev& S = a.S;
     const int nls = S.num_ls, pop = S.pop_size, NG = S.num_genes;
@@ -153,14 +154,14 @@ hipError_t launch_ls_sw(const LaunchArgs& la, const SearchDev& sp, const JobBuff
     a.L = la.lig_dev; a.G = la.grid; a.S = sp; a.B = b;
     a.lig_scale = 1.0f / sqrtf((float)la.lig_host->num_atoms);
     a.gene_scale = 1.0f / sqrtf((float)la.lig_host->num_genes);
-    const Shape sh = choose_shape(*la.lig_host, la.team_hint);
-    const int per_group = 256 / sh.tw;
+    const Shape sh = choose_shape(*la.lig_host, la.team_hint, ls_group_threads());
+    const int per_group = sh.threads / sh.tw;
     hipError_t err = hipSuccess;
     ADGPU_DISPATCH(k_ls_sw, sh, la.grid.format, {
         const int want = (n + per_group - 1) / per_group;
-        const int cap = resident_groups(reinterpret_cast<const void*>(kfn), sh.lds) / (la.share > 1 ? la.share : 1);
+        const int cap = persistent_cap(reinterpret_cast<const void*>(kfn), sh.lds, sh.threads, la.share);
         const int groups = cap > 0 && cap < want ? cap : want;
-        hipLaunchKernelGGL(kfn, dim3(groups), dim3(256), sh.lds, la.stream, a);
+        hipLaunchKernelGGL(kfn, dim3(groups), dim3(sh.threads), sh.lds, la.stream, a);
         err = hipGetLastError();
     });
     return err;

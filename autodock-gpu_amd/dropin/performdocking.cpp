@@ -4,8 +4,9 @@
 // (host/inc/performdocking.h.Cuda:65-96): get_gpu_pool, setup_gpu_for_docking,
 // finish_gpu_from_docking, docking_with_gpu and check_progress.  The host
 // preparation uses the reference's own host functions (gen_initpop_and_reflig,
-// LocalRNG, prepare_const_fields_for_gpu, AutoStop); everything that ran on
-// the GPU goes through the C ABI of include/adgpu.h (libadgpu_hip.so).
+// LocalRNG, prepare_const_fields_for_gpu, and AutoStop's table header); the
+// generations and the AutoStop tests run on the GPU through the C ABI of
+// include/adgpu.h (libadgpu_hip.so).
 // Built with the reference's host sources by the Makefile beside this file.
 #include <algorithm>
 #include <chrono>
@@ -317,53 +318,89 @@ int docking_with_gpu(const Gridinfo* mygrid, Dockpars* mypars, const Liganddata*
     auto const t2 = std::chrono::steady_clock::now();
     para_printf("\nRest of Setup time %fs\n", elapsed_seconds(t1, t2));
 
+    // AutoStop runs on the device (adgpu_job_set_autostop); ADGPU_HOST_AUTOSTOP=1
+    // tests on the host with the reference's AutoStop class instead, as the
+    // reference does (performdocking.cpp.Cuda:687-696).
+    const bool host_autostop = mypars->autostop && getenv("ADGPU_HOST_AUTOSTOP") &&
+                               strcmp(getenv("ADGPU_HOST_AUTOSTOP"), "1") == 0;
     AutoStop autostop(mypars->pop_size, mypars->num_of_runs, mypars->stopstd, mypars->as_frequency, output);
     if (mypars->autostop) {
         autostop.print_intro(mypars->num_of_generations, mypars->num_of_energy_evals);
+        if (!host_autostop)
+            check(adgpu_job_set_autostop(job, 1, mypars->stopstd, (int)mypars->as_frequency, (int64_t)min_as_evals),
+                  "adgpu_job_set_autostop", ctx);
     } else {
         para_printf("\nExecuting docking runs:\n");
         para_printf("        20%%        40%%       60%%       80%%       100%%\n");
         para_printf("---------+---------+---------+---------+---------+\n");
     }
 
-    // generation loop: without AutoStop one call runs it to the end; with
-    // AutoStop the energies are tested every as_frequency generations before
-    // the generation runs (performdocking.cpp.Cuda:687-696)
     int64_t gens = 0, total = 0;
     int curr_progress_cnt = 0;
+    bool autostopped = false;
     std::vector<int32_t> evals(nrun);
-    for (;;) {
-        check(adgpu_job_download(job, nullptr, nullptr, evals.data(), nullptr), "adgpu_job_download", ctx);
-        unsigned long total_evals = 0;
-        const double progress = check_progress(evals.data(), (int)gens, (int)mypars->num_of_energy_evals,
-                                               (int)mypars->num_of_generations, nrun, total_evals);
-        total = (int64_t)total_evals;
-        if (progress >= 100.0) break;
-        int64_t step = INT64_MAX;
-        if (mypars->autostop) {
-            const int asf = (int)mypars->as_frequency;
-            if (gens % asf == 0) {
-                check(adgpu_job_download(job, nullptr, sim_state.cpu_energies.data(), nullptr, nullptr),
-                      "adgpu_job_download", ctx);
-                if (autostop.check_if_satisfactory((int)gens, sim_state.cpu_energies.data(), total_evals))
-                    if (total_evals > min_as_evals) break;
+    if (mypars->autostop && !host_autostop) {
+        // the tests run between generations on the device; the table lines
+        // (autostop.hpp:201) are printed after every chunk of generations
+        const int64_t chunk = 20 * (int64_t)mypars->as_frequency;
+        int printed = 0;
+        for (;;) {
+            const int64_t before = gens;
+            check(adgpu_job_run_generations(job, chunk, &gens, &total), "adgpu_job_run_generations", ctx);
+            int n = 0;
+            check(adgpu_job_autostop_log(job, 0, 0, nullptr, &n), "adgpu_job_autostop_log", ctx);
+            std::vector<adgpu_autostop_record> recs(n > printed ? n - printed : 0);
+            if (!recs.empty())
+                check(adgpu_job_autostop_log(job, printed, (int)recs.size(), recs.data(), &n), "adgpu_job_autostop_log",
+                      ctx);
+            for (const auto& r : recs) {
+                para_printf("%11u | %12lu |%8.2f kcal/mol |%8.2f +/-%8.2f kcal/mol |%8i |%8.2f kcal/mol\n",
+                            (unsigned)r.generation, (unsigned long)r.total_evals / mypars->num_of_runs, r.threshold,
+                            r.average, r.stddev, r.samples, r.best_energy);
+                autostopped = autostopped || r.autostopped;
             }
-            step = asf - gens % asf;
-        } else {
-            int new_progress_cnt = (int)(progress / 2.0 + 0.5);
-            if (new_progress_cnt > 50) new_progress_cnt = 50;
-            while (curr_progress_cnt < new_progress_cnt) { curr_progress_cnt++; para_printf("*"); }
+            printed += (int)recs.size();
             fflush(stdout);
-            // progress bar updates about every 2 %
-            const double per_gen = gens > 0 ? progress / (double)gens : 0.0;
-            if (per_gen > 0.0) step = std::max<int64_t>(1, (int64_t)(2.0 / per_gen));
+            if (gens - before < chunk) break;
         }
-        int64_t g_after = 0, t_after = 0;
-        check(adgpu_job_run_generations(job, step, &g_after, &t_after), "adgpu_job_run_generations", ctx);
-        gens = g_after;
+    } else {
+        // generation loop: without AutoStop one call runs it to the end; with
+        // the host AutoStop the energies are tested every as_frequency
+        // generations before the generation runs (performdocking.cpp.Cuda:687-696)
+        for (;;) {
+            check(adgpu_job_download(job, nullptr, nullptr, evals.data(), nullptr), "adgpu_job_download", ctx);
+            unsigned long total_evals = 0;
+            const double progress = check_progress(evals.data(), (int)gens, (int)mypars->num_of_energy_evals,
+                                                   (int)mypars->num_of_generations, nrun, total_evals);
+            total = (int64_t)total_evals;
+            if (progress >= 100.0) break;
+            int64_t step = INT64_MAX;
+            if (mypars->autostop) {
+                const int asf = (int)mypars->as_frequency;
+                if (gens % asf == 0) {
+                    check(adgpu_job_download(job, nullptr, sim_state.cpu_energies.data(), nullptr, nullptr),
+                          "adgpu_job_download", ctx);
+                    if (autostop.check_if_satisfactory((int)gens, sim_state.cpu_energies.data(), total_evals))
+                        if (total_evals > min_as_evals) break;
+                }
+                step = asf - gens % asf;
+            } else {
+                int new_progress_cnt = (int)(progress / 2.0 + 0.5);
+                if (new_progress_cnt > 50) new_progress_cnt = 50;
+                while (curr_progress_cnt < new_progress_cnt) { curr_progress_cnt++; para_printf("*"); }
+                fflush(stdout);
+                // progress bar updates about every 2 %
+                const double per_gen = gens > 0 ? progress / (double)gens : 0.0;
+                if (per_gen > 0.0) step = std::max<int64_t>(1, (int64_t)(2.0 / per_gen));
+            }
+            int64_t g_after = 0, t_after = 0;
+            check(adgpu_job_run_generations(job, step, &g_after, &t_after), "adgpu_job_run_generations", ctx);
+            gens = g_after;
+        }
+        autostopped = autostop.did_stop();
     }
     profile.nev_at_stop = total / mypars->num_of_runs;
-    profile.autostopped = autostop.did_stop();
+    profile.autostopped = autostopped;
     clock_t clock_stop_docking = clock();
     if (mypars->autostop == 0) {
         while (curr_progress_cnt < 50) { curr_progress_cnt++; para_printf("*"); }
@@ -372,7 +409,27 @@ int docking_with_gpu(const Gridinfo* mygrid, Dockpars* mypars, const Liganddata*
     auto const t3 = std::chrono::steady_clock::now();
     check(adgpu_job_download(job, sim_state.cpu_populations.data(), sim_state.cpu_energies.data(),
                              sim_state.cpu_evals_of_runs.data(), nullptr), "adgpu_job_download", ctx);
-    if (mypars->autostop) autostop.output_final_stddev((int)gens, sim_state.cpu_energies.data(), total);
+    if (host_autostop) {
+        autostop.output_final_stddev((int)gens, sim_state.cpu_energies.data(), total);
+    } else if (mypars->autostop) {
+        // output_final_stddev (autostop.hpp:233-244) from the device statistics
+        adgpu_autostop_record f{};
+        check(adgpu_job_autostop_final(job, &f), "adgpu_job_autostop_final", ctx);
+        const char* rule = "------------+--------------+------------------+------------------------------+---------+-------------------\n";
+        if (f.autostopped) {
+            para_printf("%s", rule);
+            para_printf("\n%43s evaluation after reaching\n%40.2f +/-%8.2f kcal/mol combined.\n%31i samples, best inter + intra %8.2f kcal/mol.\n",
+                        "Finished", f.roll_average, f.roll_stddev, f.roll_samples, f.best_energy);
+        } else {
+            para_printf("%11u | %12lu |%8.2f kcal/mol |%8.2f +/-%8.2f kcal/mol |%8i |%8.2f kcal/mol\n", (unsigned)gens,
+                        (unsigned long)total / mypars->num_of_runs, f.threshold, f.average, f.stddev, f.samples,
+                        f.best_energy);
+            para_printf("%s", rule);
+            para_printf("\n%43s evaluation after reaching\n%30lu evaluations. Best inter + intra %8.2f kcal/mol.\n",
+                        "Finished", (unsigned long)total / mypars->num_of_runs, f.best_energy);
+        }
+        fflush(stdout);
+    }
     para_printf("\nDocking time %fs\n", elapsed_seconds(t2, t3));
 
     sim_state.myligand_reference = myligand_reference;

@@ -246,6 +246,46 @@ int  adgpu_job_best_f64(adgpu_job_t job, float* best_genotypes, float* best_ener
  * class, measured with HIP events on the job's stream (0 = gen, 1 = ls). */
 int  adgpu_job_kernel_time(adgpu_job_t job, int kernel_class, double* avg_ms, int64_t* launches);
 
+/* ---- AutoStop on the device -------------------------------------------------
+ * The reference tests its population every as_frequency generations on the
+ * host (AutoStop::check_if_satisfactory, host/inc/autostop.hpp:181-231, called
+ * at host/src/performdocking.cpp.Cuda:687-696) and leaves the generation loop
+ * when the test passes and total_evals > min_as_evals.  Here the test runs as
+ * a kernel between generations, so adgpu_job_run keeps the device busy
+ * across tests: generations enqueued behind a passed test do nothing. */
+
+/* One test, as the reference prints it (autostop.hpp:201) and as
+ * output_final_stddev reports the end (autostop.hpp:233-244). */
+typedef struct adgpu_autostop_record {
+    int64_t generation;     /* generation_cnt of the test                        */
+    int64_t total_evals;    /* sum of evals_of_runs at the test                  */
+    float   threshold;      /* threshold the test used (threshold_used)          */
+    float   average;        /* curr_avg: average energy of the best samples      */
+    float   stddev;         /* curr_std                                          */
+    int32_t samples;        /* bestN                                             */
+    float   best_energy;    /* overall_best_energy                               */
+    int32_t autostopped;    /* check_if_satisfactory's result (sticky)           */
+    int32_t stop;           /* autostopped && total_evals > min_as_evals         */
+    float   roll_average;   /* average, stddev and samples of the last four tests */
+    float   roll_stddev;
+    int32_t roll_samples;
+} adgpu_autostop_record;
+
+/* Enables (or disables, enable = 0) AutoStop for the job's next
+ * adgpu_job_run / adgpu_job_run_generations calls and resets its state (the
+ * AutoStop constructor, autostop.hpp:129-146).  stopstd in kcal/mol,
+ * as_frequency in generations, min_as_evals summed over all runs
+ * (performdocking.cpp.Cuda:465-511). */
+int  adgpu_job_set_autostop(adgpu_job_t job, int enable, float stopstd, int as_frequency, int64_t min_as_evals);
+/* One test now, on the current population, at the job's generation count;
+ * once a test has stopped the job, the job runs no more generations. */
+int  adgpu_job_autostop_check(adgpu_job_t job, adgpu_autostop_record* out);
+/* Records of the tests run so far: copies records [first, first + max) and
+ * returns the number of tests in *out_count. */
+int  adgpu_job_autostop_log(adgpu_job_t job, int first, int max, adgpu_autostop_record* out, int* out_count);
+/* The end-of-docking statistics (output_final_stddev, autostop.hpp:233-244). */
+int  adgpu_job_autostop_final(adgpu_job_t job, adgpu_autostop_record* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,30 @@ void orc_cmwc_uints(const uint32_t seed[3], uint32_t* out, int64_t n);
 void orc_initial_population(const uint32_t seed[3], int runs, int pop,
                             const float size_xyz_angstr[3], float spacing, float* out);
 
+/* ---- AutoStop (autostop_oracle.c) ----------------------------------------- */
+
+/* The members of the reference's AutoStop class (host/inc/autostop.hpp:36-58);
+ * sums = average_sd2_N[0..2] after the last test. */
+typedef struct orc_autostop {
+    int32_t  num_runs;
+    uint32_t ntop, ncream, as_frequency;
+    float    stopstd;
+    int32_t  first_time, autostopped;
+    float    threshold, threshold_used, thres_stddev, curr_avg, curr_std;
+    uint32_t roll_count, best_n;
+    float    rolling[16];
+    float    delta_energy, overall_best;
+    float    sums[3];
+} orc_autostop;
+
+void orc_autostop_init(orc_autostop* a, int pop_size, int num_runs, float stopstd, int as_frequency);
+/* check_if_satisfactory plus "total_evals > min_as_evals"; bins is scratch of
+ * 3*(pop_size+1) floats; rec may be NULL.  Returns 1 when the run stops. */
+int  orc_autostop_check(orc_autostop* a, int generation, const float* energies, int64_t total_evals,
+                        int64_t min_as_evals, float* bins, adgpu_autostop_record* rec);
+void orc_autostop_final(orc_autostop* a, int generation, const float* energies, int64_t total_evals, float* bins,
+                        adgpu_autostop_record* rec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "autodock-gpu_amd"))
-from adgpu._abi import EnergyParams, Grid, Ligand, SearchParams  # noqa: E402
+from adgpu._abi import AutostopRecord, EnergyParams, Grid, Ligand, SearchParams  # noqa: E402
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -28,6 +28,15 @@ class OrcJob(C.Structure):
     _fields_ = [("grid", C.POINTER(Grid)), ("lig", C.POINTER(Ligand)), ("ep", C.POINTER(EnergyParams)),
                 ("sp", SearchParams), ("pop_cur", _fp), ("pop_next", _fp), ("ene_cur", _fp), ("ene_next", _fp),
                 ("evals_new", _ip), ("evals_runs", _ip), ("prng", _up), ("generation", C.c_int64)]
+
+
+class OrcAutostop(C.Structure):
+    _fields_ = [("num_runs", C.c_int32), ("ntop", C.c_uint32), ("ncream", C.c_uint32), ("as_frequency", C.c_uint32),
+                ("stopstd", C.c_float), ("first_time", C.c_int32), ("autostopped", C.c_int32),
+                ("threshold", C.c_float), ("threshold_used", C.c_float), ("thres_stddev", C.c_float),
+                ("curr_avg", C.c_float), ("curr_std", C.c_float), ("roll_count", C.c_uint32),
+                ("best_n", C.c_uint32), ("rolling", C.c_float * 16), ("delta_energy", C.c_float),
+                ("overall_best", C.c_float), ("sums", C.c_float * 3)]
 
 
 _LIB = None
@@ -72,6 +81,11 @@ def lib():
         L.orc_lga_generations.argtypes = [jp, C.c_int, C.c_int, C.c_int64, C.c_int]
         L.orc_cmwc_uints.argtypes = [_up, _up, C.c_int64]
         L.orc_initial_population.argtypes = [_up, C.c_int, C.c_int, _fp, C.c_float, _fp]
+        ap, rp = C.POINTER(OrcAutostop), C.POINTER(AutostopRecord)
+        L.orc_autostop_init.argtypes = [ap, C.c_int, C.c_int, C.c_float, C.c_int]
+        L.orc_autostop_check.restype = C.c_int
+        L.orc_autostop_check.argtypes = [ap, C.c_int, _fp, C.c_int64, C.c_int64, _fp, rp]
+        L.orc_autostop_final.argtypes = [ap, C.c_int, _fp, C.c_int64, _fp, rp]
         _LIB = L
     return _LIB
 
@@ -247,6 +261,12 @@ def ref_lib():
         L.ref_ptr.restype = C.c_void_p
         L.ref_ptr.argtypes = [C.c_void_p, C.c_char_p]
         L.ref_score.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _fp]
+        L.ref_autostop_new.restype = C.c_void_p
+        L.ref_autostop_new.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int]
+        L.ref_autostop_free.argtypes = [C.c_void_p]
+        L.ref_autostop_check.restype = C.c_int
+        L.ref_autostop_check.argtypes = [C.c_void_p, C.c_int, _fp, C.c_ulong, _up]
+        L.ref_autostop_final.argtypes = [C.c_void_p, C.c_int, _fp, C.c_ulong, _up]
         _REF = L
     return _REF
 
@@ -338,3 +358,74 @@ class RefSetup:
             out["vdw"] = pa[: self.num_atoms].copy()
             out["elec"] = pa[self.num_atoms:].copy()
         return out
+
+
+# State fields compared between the oracle's AutoStop and the reference's class.
+AUTOSTOP_FLOATS = ("threshold", "threshold_used", "thres_stddev", "curr_avg", "curr_std", "delta_energy",
+                   "overall_best")
+AUTOSTOP_INTS = ("roll_count", "best_n", "first_time", "autostopped")
+
+
+class OracleAutoStop:
+    """autostop_oracle.c: the AutoStop test (host/inc/autostop.hpp:36-244)."""
+
+    def __init__(self, pop_size, num_runs, stopstd=0.15, as_frequency=5):
+        self.s = OrcAutostop()
+        self.pop, self.nrun = pop_size, num_runs
+        lib().orc_autostop_init(C.byref(self.s), pop_size, num_runs, stopstd, as_frequency)
+        self.bins = np.zeros(3 * (pop_size + 1), np.float32)
+
+    def check(self, generation, energies, total_evals, min_as_evals=0):
+        e = _f(energies).reshape(-1)
+        assert e.size == self.pop * self.nrun
+        r = AutostopRecord()
+        stop = lib().orc_autostop_check(C.byref(self.s), int(generation), e.ctypes.data_as(_fp), int(total_evals),
+                                        int(min_as_evals), self.bins.ctypes.data_as(_fp), C.byref(r))
+        return bool(stop), r.as_dict()
+
+    def final(self, generation, energies, total_evals):
+        e = _f(energies).reshape(-1)
+        r = AutostopRecord()
+        lib().orc_autostop_final(C.byref(self.s), int(generation), e.ctypes.data_as(_fp), int(total_evals),
+                                 self.bins.ctypes.data_as(_fp), C.byref(r))
+        return r.as_dict()
+
+    def state(self):
+        d = {k: np.float32(getattr(self.s, k)) for k in AUTOSTOP_FLOATS}
+        d.update({k: int(getattr(self.s, k)) for k in AUTOSTOP_INTS})
+        d["rolling"] = np.array(self.s.rolling[:], np.float32)
+        d["sums"] = np.array(self.s.sums[:], np.float32)
+        return d
+
+
+class RefAutoStop:
+    """The reference's own AutoStop class (oracle/_ref build)."""
+
+    def __init__(self, pop_size, num_runs, stopstd=0.15, as_frequency=5):
+        self.h = ref_lib().ref_autostop_new(pop_size, num_runs, stopstd, as_frequency)
+        self.w = np.zeros(33, np.uint32)
+
+    def __del__(self):
+        try:
+            ref_lib().ref_autostop_free(self.h)
+        except Exception:
+            pass
+
+    def check(self, generation, energies, total_evals):
+        e = _f(energies).reshape(-1)
+        return bool(ref_lib().ref_autostop_check(self.h, int(generation), e.ctypes.data_as(_fp), int(total_evals),
+                                                 self.w.ctypes.data_as(_up)))
+
+    def final(self, generation, energies, total_evals):
+        e = _f(energies).reshape(-1)
+        ref_lib().ref_autostop_final(self.h, int(generation), e.ctypes.data_as(_fp), int(total_evals),
+                                     self.w.ctypes.data_as(_up))
+
+    def state(self):
+        f = self.w.view(np.float32)
+        d = dict(zip(AUTOSTOP_FLOATS, [np.float32(x) for x in f[:7]]))
+        d["rolling"] = f[7:23].copy()
+        d["sums"] = f[23:26].copy()
+        d.update(roll_count=int(self.w[26]), best_n=int(self.w[27]), first_time=int(self.w[28]),
+                 autostopped=int(self.w[29]))
+        return d

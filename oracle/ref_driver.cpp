@@ -10,7 +10,8 @@
 //   - initial population and PRNG seeds (docking_with_gpu,
 //     host/src/performdocking.cpp.Cuda:306-317),
 //   - CPU scoring of a genotype exactly as make_resfiles does it
-//     (host/src/processresult.cpp:430-466).
+//     (host/src/processresult.cpp:430-466),
+//   - the AutoStop test (host/inc/autostop.hpp).
 #include <cstring>
 #include <cstdio>
 #include <vector>
@@ -182,3 +183,56 @@ int ref_matrix(void* h, const char* name, void* out) {
     return 1;
 }
 }
+
+// The reference's AutoStop class (host/inc/autostop.hpp), with its private
+// members opened (declared as a struct) so tests can compare the whole state
+// after every test.  Its own includes are included first.
+#include <cmath>
+#include <stdio.h>
+#include <stdlib.h>
+#include "miscellaneous.h"
+#define class struct
+#include "autostop.hpp"
+#undef class
+
+extern "C" {
+
+void* ref_autostop_new(int pop_size, int num_runs, float stopstd, int as_frequency) {
+    return new AutoStop(pop_size, num_runs, stopstd, as_frequency, nullptr);
+}
+
+void ref_autostop_free(void* h) { delete (AutoStop*)h; }
+
+// State after a call: 33 words, floats as floats and counters as uint32:
+// threshold, threshold_used, thres_stddev, curr_avg, curr_std, delta_energy,
+// overall_best_energy, rolling[16], average_sd2_N[0..2], roll_count, bestN,
+// first_time, autostopped.
+static void ref_autostop_state(const AutoStop* a, uint32_t* out) {
+    const float f[7] = {a->threshold, a->threshold_used, a->thres_stddev, a->curr_avg, a->curr_std,
+                        a->delta_energy, a->overall_best_energy};
+    std::memcpy(out, f, sizeof(f));
+    std::memcpy(out + 7, a->rolling.data(), 16 * sizeof(float));
+    std::memcpy(out + 23, a->average_sd2_N.data(), 3 * sizeof(float));
+    out[26] = a->roll_count;
+    out[27] = a->bestN;
+    out[28] = a->first_time;
+    out[29] = a->autostopped;
+    out[30] = a->Ntop;
+    out[31] = a->Ncream;
+    out[32] = a->as_frequency;
+}
+
+int ref_autostop_check(void* h, int generation, const float* energies, unsigned long total_evals, uint32_t* state) {
+    AutoStop* a = (AutoStop*)h;
+    const bool r = a->check_if_satisfactory(generation, energies, total_evals);
+    if (state) ref_autostop_state(a, state);
+    return r ? 1 : 0;
+}
+
+void ref_autostop_final(void* h, int generation, const float* energies, unsigned long total_evals, uint32_t* state) {
+    AutoStop* a = (AutoStop*)h;
+    a->output_final_stddev(generation, energies, total_evals);
+    if (state) ref_autostop_state(a, state);
+}
+
+}  // extern "C"

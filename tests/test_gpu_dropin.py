@@ -74,3 +74,25 @@ def test_cli_other_local_searches_with_autostop(tmp_path, lsmet):
                      "--heuristics", "0", "--autostop", "1", "--lsmet", lsmet)
     e = _docked_energies(open(res + ".dlg", encoding="latin-1").read())
     assert len(e) == 6 and np.all(np.isfinite(e)), (lsmet, e)
+
+
+@pytest.mark.parametrize("lsmet", ["ad", "sw"])
+def test_cli_device_autostop_equals_reference_host_autostop(tmp_path, lsmet):
+    """The AutoStop test on the device against the reference's own AutoStop
+    class on the host (ADGPU_HOST_AUTOSTOP=1): same table, same stop, same
+    docked poses."""
+    args = ("--nrun", "8", "--psize", "60", "--nev", "400000", "--seed", "4,4,4", "--heuristics", "0",
+            "--autostop", "1", "--asfreq", "5", "--stopstd", "0.5", "--lsmet", lsmet)
+    r1, out1 = _dock(tmp_path, "dev", *args)
+    env = dict(os.environ, ADGPU_HOST_AUTOSTOP="1")
+    fld, pdbqt = _inputs(tmp_path)
+    r2 = os.path.join(str(tmp_path), "host")
+    p = subprocess.run([CLI, "--ffile", fld, "--lfile", pdbqt, "--resnam", r2] + list(args), capture_output=True,
+                       text=True, timeout=240, env=env)
+    assert p.returncode == 0, p.stdout[-3000:]
+    table = lambda t: [ln for ln in t.splitlines() if re.match(r"^\s+\d+ \|", ln) or "Finished" in ln
+                       or "kcal/mol combined" in ln or "samples, best" in ln or "evaluations. Best" in ln]
+    assert table(out1) == table(p.stdout) and len(table(out1)) > 2
+    d1 = open(r1 + ".dlg", encoding="latin-1").read()
+    d2 = open(r2 + ".dlg", encoding="latin-1").read()
+    assert np.array_equal(_docked_energies(d1), _docked_energies(d2))
