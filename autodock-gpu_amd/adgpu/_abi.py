@@ -153,11 +153,12 @@ _LIB = None
 
 
 def load_library(path: str | None = None):
-    """Loads the HIP library; raises if it has not been built."""
+    """Loads the HIP library (ADGPU_LIBRARY names another build of it);
+    raises if it has not been built."""
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
-    p = path or LIB_PATH
+    p = path or os.environ.get("ADGPU_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError("HIP library %s is missing: run __graft_entry__.build() (or python -m adgpu.build)" % p)
     lib = C.CDLL(p)

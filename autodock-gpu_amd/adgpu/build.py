@@ -36,9 +36,9 @@ def flags():
             "-I", os.path.join(REPO, "include"), "-Wno-unused-command-line-argument"]
 
 
-def _compile(src: str, tmp: str, verbose: bool) -> str:
+def _compile(src: str, tmp: str, verbose: bool, defines=()) -> str:
     obj = os.path.join(tmp, src + ".o")
-    cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc()] + flags() + ["-D%s" % d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".hip"):
         cmd[1:1] = ["-x", "hip"]
     if verbose:
@@ -47,19 +47,21 @@ def _compile(src: str, tmp: str, verbose: bool) -> str:
     return obj
 
 
-def build(verbose: bool = False, jobs: int = 0) -> str:
-    tmp = os.path.join(PKG, "_build")
+def build(verbose: bool = False, jobs: int = 0, defines=(), out: str = OUT, tmp: str | None = None) -> str:
+    """Compiles every source and links `out`.  defines: extra -D macros
+    (e.g. ADGPU_WAVES_PER_EU=5) for a variant build written elsewhere."""
+    tmp = tmp or os.path.join(PKG, "_build")
     os.makedirs(tmp, exist_ok=True)
     jobs = jobs or max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1), 16))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, tmp, verbose), SOURCES))
-    tmp_out = OUT + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, tmp, verbose, defines), SOURCES))
+    tmp_out = out + ".tmp"
     cmd = [hipcc(), "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp_out] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp_out, OUT)
-    return OUT
+    os.replace(tmp_out, out)
+    return out
 
 
 if __name__ == "__main__":

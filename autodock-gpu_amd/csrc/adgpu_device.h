@@ -33,7 +33,11 @@ namespace adgpu {
 
 #define DEV __device__ __forceinline__
 // 256-thread workgroups; at least 4 wavefronts per SIMD (<= 128 VGPRs)
-#define ADGPU_GROUP_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+// Waves per SIMD the register budget is sized for (4: at most 128 VGPRs).
+#ifndef ADGPU_WAVES_PER_EU
+#define ADGPU_WAVES_PER_EU 4
+#endif
+#define ADGPU_GROUP_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADGPU_WAVES_PER_EU)))
 
 // common/calcenergy_basic.h
 constexpr float kDegToRad = 0.0174533f;
@@ -71,19 +75,22 @@ struct Team {
     DEV static int slot() { return lane_id() / TW; }
     // Sum over the team; every lane of the team receives the same bits.
     // Rows of 16 lanes: xor 1, xor 2 (quad_perm), half-row mirror, row mirror;
-    // then the row sums are combined through readlane.
+    // then the row sums are combined with gfx950's row swaps: permlane16_swap
+    // gives rows 2k and 2k+1 both (row 2k, row 2k+1), so both add
+    // row 2k + row 2k+1 in that order; permlane32_swap does the same for the
+    // two halves of the wavefront.
     DEV static float sum(float v) {
         v += dpp<0xB1>(v);
         v += dpp<0x4E>(v);
         v += dpp<0x141>(v);
         v += dpp<0x140>(v);
-        if constexpr (TW == 32) {
-            float a = rdl(v, 0) + rdl(v, 16);
-            float b = rdl(v, 32) + rdl(v, 48);
-            return lane_id() < 32 ? a : b;
-        } else {
-            return (rdl(v, 0) + rdl(v, 16)) + (rdl(v, 32) + rdl(v, 48));
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+        v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        if constexpr (TW == 64) {
+            const auto h = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+            v = __uint_as_float(h[0]) + __uint_as_float(h[1]);
         }
+        return v;
     }
     // Value of team lane k (k uniform).
     DEV static float bcast(float v, int k) {

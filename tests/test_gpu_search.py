@@ -147,3 +147,48 @@ def test_lga_statistical_parity_with_oracle(gpu_ctx, c1g):
     assert np.all(np.isfinite(best_gpu))
     # medians of 24 runs; spread of the per-run best is a few kcal/mol
     assert abs(np.median(best_gpu) - np.median(best_orc)) < 1.0, (np.median(best_gpu), np.median(best_orc))
+
+
+_SHAPE_CHILD = r"""
+import sys, numpy as np
+sys.path[:0] = [{paths}]
+import adgpu, helpers, oracle as O
+from adgpu._abi import search_params
+case = helpers.synthetic_case(30, 6, 7, 11, npts=60)
+ctx = adgpu.Context(0)
+mid = ctx.maps_upload(case.packed, case.size, case.maps.num_map_atypes, case.spacing)
+lid = ctx.ligand_upload(case.tables, case.ep)
+sp = search_params(num_runs=6, pop_size=40, max_evals=40000, ls="{ls}", spacing=case.spacing)
+init = O.initial_population((3, 1), 6, 40, case.maps.size_angstr, case.spacing)
+job = ctx.job(mid, lid, sp, init, O.cmwc_uints((3, 1), 6 * 40 * 64))
+job.initpop()
+job.run()
+st = job.download()
+np.savez(sys.argv[1], pop=st["pop"], ene=st["ene"], ev=st["evals_runs"])
+"""
+
+
+@pytest.mark.parametrize("ls", ["ad", "sw", "fire"])
+def test_results_do_not_depend_on_launch_shape(gpu_ctx, c1g, ls, tmp_path):
+    """The persistent local searches in 64-thread workgroups, one run group,
+    or each group on the whole device give bit-identical jobs: an item's
+    result depends only on its own LCG streams."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    repo = os.path.dirname(here)
+    paths = ", ".join(repr(p) for p in (os.path.join(repo, "autodock-gpu_amd"), os.path.join(repo, "oracle"), here))
+    code = _SHAPE_CHILD.format(paths=paths, ls=ls)
+    outs = []
+    for k, env in enumerate(({}, {"ADGPU_LS_BLOCK": "64"}, {"ADGPU_RUN_GROUPS": "1", "ADGPU_LS_BLOCK": "128"},
+                             {"ADGPU_LS_SHARE": "0"})):
+        f = os.path.join(str(tmp_path), "r%d.npz" % k)
+        print("launch shape", env, flush=True)
+        p = subprocess.run([sys.executable, "-c", code, f], env=dict(os.environ, **env), capture_output=True,
+                           text=True, timeout=100)
+        assert p.returncode == 0, (env, p.stderr[-2000:])
+        outs.append(np.load(f))
+    for o in outs[1:]:
+        for k in ("pop", "ene", "ev"):
+            assert np.array_equal(o[k], outs[0][k]), k
