@@ -210,6 +210,12 @@ class Job:
         self._c(self.lib.adgpu_job_kernel_time(self.h, cls, C.byref(ms), C.byref(n)), "adgpu_job_kernel_time")
         return ms.value, n.value
 
+    def kernel_busy(self, cls):
+        """Union of the launch intervals of one kernel class (ms) and the launches."""
+        ms, n = C.c_double(), C.c_int64()
+        self._c(self.lib.adgpu_job_kernel_busy(self.h, cls, C.byref(ms), C.byref(n)), "adgpu_job_kernel_busy")
+        return ms.value, n.value
+
     def set_autostop(self, enable=True, stopstd=0.15, as_frequency=5, min_as_evals=0):
         """AutoStop tested on the device every as_frequency generations."""
         self._c(self.lib.adgpu_job_set_autostop(self.h, int(bool(enable)), float(stopstd), int(as_frequency),

@@ -189,6 +189,7 @@ def load_library(path: str | None = None):
     lib.adgpu_job_download.argtypes = [vp, _fp, _fp, _ip, _up]
     lib.adgpu_job_upload.argtypes = [vp, _fp, _fp, _up]
     lib.adgpu_job_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.adgpu_job_kernel_busy.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     rp = C.POINTER(AutostopRecord)
     lib.adgpu_job_set_autostop.argtypes = [vp, C.c_int, C.c_float, C.c_int, C.c_int64]
     lib.adgpu_job_autostop_check.argtypes = [vp, rp]
@@ -203,5 +204,5 @@ EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "a
             "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses",
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",
             "adgpu_job_kernel_ls", "adgpu_job_kernel_sum_and_swap", "adgpu_job_generation", "adgpu_job_run",
-            "adgpu_job_download", "adgpu_job_upload", "adgpu_job_kernel_time", "adgpu_job_set_autostop",
+            "adgpu_job_download", "adgpu_job_upload", "adgpu_job_kernel_time", "adgpu_job_kernel_busy", "adgpu_job_set_autostop",
             "adgpu_job_autostop_check", "adgpu_job_autostop_log", "adgpu_job_autostop_final"]

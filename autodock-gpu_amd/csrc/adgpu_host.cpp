@@ -90,6 +90,11 @@ struct adgpu_job {
     int64_t generation = 0;
     double kernel_ms[2] = {0.0, 0.0};
     int64_t kernel_n[2] = {0, 0};
+    // launch intervals (ms after ev_origin) per kernel class, for the time the
+    // device spent in that class when launches of several groups overlap
+    hipEvent_t ev_origin = nullptr;
+    bool origin_set = false;
+    std::vector<std::pair<double, double>> spans[2];
     // AutoStop on the device (k_autostop.hip)
     int32_t* d_stop = nullptr;          // raised by a passed test; every generation kernel then returns
     int32_t* h_stop = nullptr;          // pinned copy, written after each test
@@ -534,10 +539,14 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     if (j->sd.num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
-    // run groups: ADGPU_RUN_GROUPS streams (default 2: with the context's
-    // stream that is 3 of HIP's 4 hardware queues per process; groups whose
-    // streams share a hardware queue would serialize)
+    // run groups: ADGPU_RUN_GROUPS streams.  Groups whose streams share a
+    // hardware queue serialize, so the default follows the HIP runtime's
+    // hardware queues per process (GPU_MAX_HW_QUEUES, 4 unless set): 2 groups
+    // (with the context's stream, 3 of 4 queues), or 4 groups when at least
+    // 6 queues are available (the faster setting on MI355X)
     int ng = 2;
+    if (const char* q = std::getenv("GPU_MAX_HW_QUEUES"))
+        if (std::atoi(q) >= 6) ng = 4;
     if (const char* e = std::getenv("ADGPU_RUN_GROUPS")) ng = std::atoi(e);
     ng = std::max(1, std::min(ng, std::min(nrun, 8)));
     bool ok = hipMalloc(&j->pop[0], ne * kGenes * sizeof(float)) == hipSuccess &&
@@ -551,6 +560,7 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
               hipMalloc(&j->d_stop, 16 * sizeof(int32_t)) == hipSuccess &&
               hipHostMalloc(&j->h_stop, 16 * sizeof(int32_t)) == hipSuccess &&
               hipEventCreateWithFlags(&j->ev_test, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreate(&j->ev_origin) == hipSuccess &&
               hipHostMalloc(&j->h_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess;
     if (ok && sp->ls_method == ADGPU_LS_FIRE) {
         const size_t nls = (size_t)nrun * (j->sd.num_ls > 0 ? j->sd.num_ls : 1);
@@ -606,6 +616,7 @@ int adgpu_job_destroy(adgpu_job_t j) {
     (void)hipFree(j->d_stop); (void)hipFree(j->d_as); (void)hipFree(j->d_as_bins);
     (void)hipFree(j->d_as_log); (void)hipFree(j->d_as_out);
     if (j->ev_test) (void)hipEventDestroy(j->ev_test);
+    if (j->ev_origin) (void)hipEventDestroy(j->ev_origin);
     if (j->h_stop) (void)hipHostFree(j->h_stop);
     if (j->h_ring) (void)hipHostFree(j->h_ring);
     delete j;
@@ -762,12 +773,18 @@ static int complete_generation(adgpu_job_t j, int k) {
     int64_t t = 0;
     for (int r = g.r0; r < g.r1; ++r) t += j->h_ring[(size_t)slot * j->sp.num_runs + r];
     g.totals.push_back(t);
-    float ms = 0.f;
+    float ms = 0.f, a = 0.f, b = 0.f;
     if (hipEventElapsedTime(&ms, g.ev_gen[0][slot], g.ev_gen[1][slot]) == hipSuccess) {
         j->kernel_ms[0] += ms; j->kernel_n[0]++;
+        if (hipEventElapsedTime(&a, j->ev_origin, g.ev_gen[0][slot]) == hipSuccess &&
+            hipEventElapsedTime(&b, j->ev_origin, g.ev_gen[1][slot]) == hipSuccess)
+            j->spans[0].emplace_back(a, b);
     }
     if (g.ls_used[slot] && hipEventElapsedTime(&ms, g.ev_ls[0][slot], g.ev_ls[1][slot]) == hipSuccess) {
         j->kernel_ms[1] += ms; j->kernel_n[1]++;
+        if (hipEventElapsedTime(&a, j->ev_origin, g.ev_ls[0][slot]) == hipSuccess &&
+            hipEventElapsedTime(&b, j->ev_origin, g.ev_ls[1][slot]) == hipSuccess)
+            j->spans[1].emplace_back(a, b);
     }
     g.done++;
     return ADGPU_OK;
@@ -841,6 +858,11 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
         g.parity0 = g.parity;
     }
     if (j->stopped) max_new = 0;   // a passed AutoStop test ended the job
+    if (!j->origin_set) {          // time origin of the launch intervals
+        HIPCHK(c, hipEventRecord(j->ev_origin, j->groups[0].stream));
+        HIPCHK(c, hipEventSynchronize(j->ev_origin));
+        j->origin_set = true;
+    }
     // most evals one run can add in one generation: K4 offspring plus every LS
     // slot at its iteration cap (two evaluations per Solis-Wets iteration)
     const int64_t emax = (int64_t)(j->sp.pop_size - 1) +
@@ -1095,6 +1117,26 @@ int adgpu_job_best_f64(adgpu_job_t j, float* best_geno, float* best_e, int32_t* 
     if (best_geno) std::memcpy(best_geno, geno.data(), geno.size() * sizeof(float));
     if (!total && !inter && !intra) return ADGPU_OK;
     return adgpu_rescore_f64(j->ctx, j->map_id, j->ligand_id, geno.data(), nrun, total, inter, intra);
+}
+
+int adgpu_job_kernel_busy(adgpu_job_t j, int cls, double* busy_ms, int64_t* launches) {
+    if (!j || cls < 0 || cls > 1) return ADGPU_ERR_ARG;
+    std::vector<std::pair<double, double>> v = j->spans[cls];
+    std::sort(v.begin(), v.end());
+    double busy = 0.0, s0 = 0.0, e0 = -1.0;
+    for (const auto& iv : v) {
+        if (iv.first > e0) {
+            if (e0 > s0) busy += e0 - s0;
+            s0 = iv.first;
+            e0 = iv.second;
+        } else if (iv.second > e0) {
+            e0 = iv.second;
+        }
+    }
+    if (e0 > s0) busy += e0 - s0;
+    if (busy_ms) *busy_ms = busy;
+    if (launches) *launches = (int64_t)v.size();
+    return ADGPU_OK;
 }
 
 int adgpu_job_kernel_time(adgpu_job_t j, int cls, double* avg_ms, int64_t* launches) {

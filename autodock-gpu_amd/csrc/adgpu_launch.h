@@ -49,14 +49,16 @@ inline int ls_group_threads() {
 }
 
 // Resident workgroups a persistent launch may take when `share` launches run
-// side by side (ADGPU_LS_SHARE=0 lets every launch take the whole device).
+// side by side: 1/share of the device.  ADGPU_LS_SHARE=k divides by k
+// instead (0: every launch may take the whole device).
 inline int persistent_cap(const void* kfn, size_t lds, int threads, int share) {
-    static const bool split = [] {
+    static const int fixed = [] {
         const char* e = std::getenv("ADGPU_LS_SHARE");
-        return !(e && std::atoi(e) == 0);
+        return e ? std::atoi(e) : -1;
     }();
     const int n = resident_groups(kfn, lds, threads);
-    return split && share > 1 ? n / share : n;
+    const int d = fixed >= 0 ? fixed : share;
+    return d > 1 ? n / d : n;
 }
 
 }  // namespace adgpu

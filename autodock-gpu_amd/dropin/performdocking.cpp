@@ -110,6 +110,9 @@ void to_adgpu_ligand(const Liganddata& lig, const kernelconstant_interintra& ki,
 }  // namespace
 
 std::vector<int> get_gpu_pool() {
+    // eight hardware queues per process (HIP's default is 4) before the HIP
+    // runtime starts, so the library runs each job in four run groups
+    setenv("GPU_MAX_HW_QUEUES", "8", 0);
     int n = 0;
     if (adgpu_device_count(&n) != ADGPU_OK || n <= 0) {
         printf("No HIP devices found, exiting.\n");

@@ -26,6 +26,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "autodock-gpu_amd"))
 
+
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 
 # Memory traffic (L2 <-> memory) of the local-search kernel per local-search
@@ -34,7 +35,7 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 # copy kernel (FETCH_SIZE reports 0.5 GiB for the 1 GiB read: factor 2.0;
 # WRITE_SIZE reports 1 GiB: factor 1.0).  Summary and commands:
 # profiles/k_ls_ad_c1_traffic.md.  Keyed by (maps, nrun, pop, ls).
-TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 1178.0}
+TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 977.0}
 
 
 def parse():
@@ -50,6 +51,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--team-width", type=int, default=0, choices=[0, 32, 64])
+    # hardware queues of this process, set before the HIP runtime starts (HIP's
+    # default is 4): with 8 the library runs the job's runs in four groups on
+    # four streams, each on its own queue
+    ap.add_argument("--hw-queues", type=int, default=8, choices=range(1, 33), metavar="1..32")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy kernel (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -133,6 +138,7 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
+    os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one, and the docking "
               "path shards by ligand/run with no inter-GPU exchange (see the sharding note "
@@ -150,7 +156,7 @@ def main():
     n_inter = int(tables.num_atoms - int(np.sum(tables.ignore_inter)))
     bytes_per_eval = n_inter * 3 * 8 * {"f32": 4, "f16": 2, "bf16": 2}[a.maps]
 
-    lat, evals_list, gens_list, ls_ms, ls_n, gen_ms = [], [], [], [], [], []
+    lat, evals_list, gens_list, ls_ms, ls_n, gen_ms, ls_busy = [], [], [], [], [], [], []
     ls_evals_list = []
     for step in range(a.warmup + a.steps):
         init, seeds = initial_state(1000 + step, a.nrun, a.pop, maps)
@@ -160,6 +166,7 @@ def main():
         gens, total = job.run()
         dt = time.perf_counter() - t0
         ms_ls, n_ls = job.kernel_time(1)
+        busy_ls, _ = job.kernel_busy(1)
         ms_gen, n_gen = job.kernel_time(0)
         st = job.download()
         job.destroy()
@@ -169,6 +176,7 @@ def main():
             gens_list.append(gens)
             ls_ms.append(ms_ls)
             ls_n.append(n_ls)
+            ls_busy.append(busy_ls)
             gen_ms.append(ms_gen)
             # LS evaluations: total minus K1 (nrun*pop) and K4 (nrun*(pop-1) per generation)
             ls_evals_list.append(total - a.nrun * a.pop - gens * a.nrun * (a.pop - 1))
@@ -180,12 +188,19 @@ def main():
     evals_per_s = evals / t
     lat_arr = np.array(lat)
     # roofline of the dominant kernel: algorithmic map-gather bytes per LS launch
-    # divided by the LS launch's average duration (HIP events on the job stream)
+    # divided by the LS launch's duration, from HIP events on the launches'
+    # streams.  The run groups' launches overlap, so the duration per launch is
+    # the device time during which any LS launch ran (the union of the launch
+    # intervals) divided by the number of launches; each launch's own
+    # event-to-event time, which includes co-running launches, is reported as
+    # ls_avg_ms.
     tot_ls_ms = float(np.sum(np.array(ls_ms) * np.array(ls_n)))
     ls_evals = float(np.sum(ls_evals_list))
-    avg_ls_ms = tot_ls_ms / max(1, int(np.sum(ls_n)))
-    bytes_per_launch = ls_evals * bytes_per_eval / max(1, int(np.sum(ls_n)))
-    achieved = bytes_per_launch / (avg_ls_ms * 1e-3) / 1e9
+    n_launch = max(1, int(np.sum(ls_n)))
+    avg_ls_ms = tot_ls_ms / n_launch
+    eff_ls_ms = float(np.sum(ls_busy)) / n_launch
+    bytes_per_launch = ls_evals * bytes_per_eval / n_launch
+    achieved = bytes_per_launch / (eff_ls_ms * 1e-3) / 1e9
     tkey = (a.maps, a.nrun, a.pop, a.ls)
     traffic = (TRAFFIC_BYTES_PER_LS_EVAL[tkey] * ls_evals / max(1, int(np.sum(ls_n)))
                if tkey in TRAFFIC_BYTES_PER_LS_EVAL else None)
@@ -208,14 +223,16 @@ def main():
         "data": "synthetic (seeded ligand and maps generated in-process)",
         "config": {"workload": "LGA ADADELTA, 1 ligand of 30 atoms / 6 torsions, 60^3 maps at 0.375 A (%s), "
                                "nrun %d, pop %d, %d evals per run" % (a.maps.upper(), a.nrun, a.pop, a.evals),
-                   "nrun": a.nrun, "pop_size": a.pop, "evals_per_run": a.evals, "ls": a.ls, "maps": a.maps},
+                   "nrun": a.nrun, "pop_size": a.pop, "evals_per_run": a.evals, "ls": a.ls, "maps": a.maps,
+                   "hw_queues": a.hw_queues},
         "ligands_per_hour": 3600.0 / float(np.mean(lat_arr)),
         "latency_s": {"p50": float(np.percentile(lat_arr, 50)), "p99": float(np.percentile(lat_arr, 99))},
         "generations_per_ligand": float(np.mean(gens_list)),
-        "kernels": {"ls_avg_ms": avg_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
-                    "ls_launches": int(np.sum(ls_n)), "ls_share_of_wall": tot_ls_ms / 1000.0 / t},
+        "kernels": {"ls_avg_ms": avg_ls_ms, "ls_busy_ms_per_launch": eff_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
+                    "ls_launches": int(np.sum(ls_n)), "ls_busy_share_of_wall": float(np.sum(ls_busy)) / 1000.0 / t},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "duration": "LS busy time (union of launch intervals) / launches",
                      "bytes_per_eval": bytes_per_eval, "aggregate_achieved": aggregate,
                      "aggregate_frac": aggregate / HBM_PEAK_GBS},
     }

@@ -245,6 +245,11 @@ int  adgpu_job_best_f64(adgpu_job_t job, float* best_genotypes, float* best_ener
 /* Average device time in milliseconds of the last launches of one kernel
  * class, measured with HIP events on the job's stream (0 = gen, 1 = ls). */
 int  adgpu_job_kernel_time(adgpu_job_t job, int kernel_class, double* avg_ms, int64_t* launches);
+/* Device time during which at least one launch of the class ran (the union
+ * of the launch intervals of all run groups since the job's first run call),
+ * and the number of launches.  No counterpart in the reference: measurement
+ * for callers that time the kernels of overlapping run groups. */
+int  adgpu_job_kernel_busy(adgpu_job_t job, int kernel_class, double* busy_ms, int64_t* launches);
 
 /* ---- AutoStop on the device -------------------------------------------------
  * The reference tests its population every as_frequency generations on the

@@ -33,19 +33,43 @@ def energy_bar(orc, pose):
     return ENERGY_RTOL * max(1.0, s), COORD_EPS * orc.sensitivity(pose)
 
 
-def check_energies(orc, poses, e_gpu, e_ref=None):
-    """Asserts the per-pose energy bar; returns the strict-bar share."""
+# An atom within EDGE_EPS grid units of the in-grid boundary (0 <= x < n-1,
+# calcenergy.cl:293-301) is inside or outside the maps depending on the last
+# bits of its FP32 coordinates (FP32 pose arithmetic moves a coordinate near
+# 100 grid units by up to ~1e-4): such a pose has two valid energies (the
+# interpolated one and +2^24).  These poses are left out of the per-pose bar;
+# they must stay rare.
+EDGE_EPS = 1e-3
+EDGE_SHARE = 0.02
+
+
+def on_grid_edge(orc, pose):
+    xyz = orc.coords64(pose)
+    g = orc.cg.s
+    hi = np.array([g.size_x - 1, g.size_y - 1, g.size_z - 1], np.float64)
+    return bool(np.min(np.minimum(np.abs(xyz), np.abs(hi - xyz))) < EDGE_EPS)
+
+
+def check_energies(orc, poses, e_gpu, e_ref=None, edge_share=EDGE_SHARE):
+    """Asserts the per-pose energy bar; returns the strict-bar share.
+    edge_share: largest share of grid-edge poses accepted (a converged
+    population may sit on the edge with many copies of one pose)."""
     strict = 0
+    edge = 0
     bad = []
     for i, p in enumerate(poses):
         eo = orc.energy32(p)[0] if e_ref is None else e_ref[i]
         t1, t2 = energy_bar(orc, p)
         d = abs(float(e_gpu[i]) - float(eo))
+        if not d <= t1 + t2 and np.isfinite(e_gpu[i]) and on_grid_edge(orc, p):
+            edge += 1
+            continue
         strict += d <= t1
         if not d <= t1 + t2:
             bad.append((i, float(e_gpu[i]), float(eo), t1, t2))
     assert not bad, bad[:5]
-    share = strict / max(1, len(poses))
+    assert edge <= edge_share * len(poses), edge
+    share = strict / max(1, len(poses) - edge)
     assert share >= STRICT_SHARE, share
     return share
 
