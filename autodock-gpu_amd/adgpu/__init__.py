@@ -64,7 +64,8 @@ class Context:
         return tot, inter, intra
 
     def set_team_width(self, width: int):
-        """0 = per ligand, 32 = two individuals per wavefront, 64 = one."""
+        """0 = per ligand, 16 = four individuals per wavefront (ligands of at
+        most 32 atoms), 32 = two, 64 = one."""
         self._check(self.lib.adgpu_set_team_width(self.h, int(width)), "adgpu_set_team_width")
 
     def close(self):

@@ -68,7 +68,7 @@ DEV int rdl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 template <int TW>
 struct Team {
-    static_assert(TW == 32 || TW == 64, "team width");
+    static_assert(TW == 16 || TW == 32 || TW == 64, "team width");
     static constexpr int kPerWave = 64 / TW;
     static constexpr int kGPL = 64 / TW;          // genes per lane
     DEV static int lane() { return lane_id() & (TW - 1); }
@@ -79,11 +79,13 @@ struct Team {
     // gives rows 2k and 2k+1 both (row 2k, row 2k+1), so both add
     // row 2k + row 2k+1 in that order; permlane32_swap does the same for the
     // two halves of the wavefront.
+    // A 16-lane team is one row: the four DPP steps give its sum.
     DEV static float sum(float v) {
         v += dpp<0xB1>(v);
         v += dpp<0x4E>(v);
         v += dpp<0x141>(v);
         v += dpp<0x140>(v);
+        if constexpr (TW == 16) return v;
         const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
         v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
         if constexpr (TW == 64) {
@@ -94,12 +96,26 @@ struct Team {
     }
     // Value of team lane k (k uniform).
     DEV static float bcast(float v, int k) {
-        if constexpr (TW == 32) return lane_id() < 32 ? rdl(v, k) : rdl(v, 32 + k);
-        else return rdl(v, k);
+        if constexpr (TW == 16) {
+            const float a = rdl(v, k), b = rdl(v, 16 + k), c = rdl(v, 32 + k), d = rdl(v, 48 + k);
+            const int s = lane_id() >> 4;
+            return s == 0 ? a : s == 1 ? b : s == 2 ? c : d;
+        } else if constexpr (TW == 32) {
+            return lane_id() < 32 ? rdl(v, k) : rdl(v, 32 + k);
+        } else {
+            return rdl(v, k);
+        }
     }
     DEV static int bcast_i(int v, int k) {
-        if constexpr (TW == 32) return lane_id() < 32 ? rdl_i(v, k) : rdl_i(v, 32 + k);
-        else return rdl_i(v, k);
+        if constexpr (TW == 16) {
+            const int a = rdl_i(v, k), b = rdl_i(v, 16 + k), c = rdl_i(v, 32 + k), d = rdl_i(v, 48 + k);
+            const int s = lane_id() >> 4;
+            return s == 0 ? a : s == 1 ? b : s == 2 ? c : d;
+        } else if constexpr (TW == 32) {
+            return lane_id() < 32 ? rdl_i(v, k) : rdl_i(v, 32 + k);
+        } else {
+            return rdl_i(v, k);
+        }
     }
     // True when any lane of the wavefront has p set.
     DEV static bool any_in_wave(bool p) { return __ballot(p) != 0ull; }
@@ -969,12 +985,14 @@ struct Shape {
 constexpr size_t kLdsBudget = 64 * 1024;
 
 // Chooses the team width and atom chunks for a ligand: 32-lane teams (two
-// individuals per wavefront) up to 128 atoms, 64-lane teams beyond.
+// individuals per wavefront) up to 128 atoms, 64-lane teams beyond; 16-lane
+// teams (four per wavefront) on request for ligands of at most 32 atoms.
 inline Shape choose_shape(const LigandDev& L, int team_width_hint, int threads = 256) {
     Shape s{};
     s.threads = threads;
     const int N = L.num_atoms;
     int tw = team_width_hint == 64 ? 64 : (team_width_hint == 32 ? 32 : (N <= 128 ? 32 : 64));
+    if (team_width_hint == 16 && N <= 32) tw = 16;
     if (tw == 32 && N > 128) tw = 64;
     s.tw = tw;
     const int per = (N + tw - 1) / tw;
@@ -1004,12 +1022,16 @@ inline Shape choose_shape(const LigandDev& L, int team_width_hint, int threads =
             if ((shape).pl) mf_sel(tw_, nc_, std::true_type{});                                 \
             else mf_sel(tw_, nc_, std::false_type{});                                           \
         };                                                                                      \
+        using I16 = std::integral_constant<int, 16>;                                            \
         using I32 = std::integral_constant<int, 32>;                                            \
         using I64 = std::integral_constant<int, 64>;                                            \
         using N1 = std::integral_constant<int, 1>;                                              \
         using N2 = std::integral_constant<int, 2>;                                              \
         using N4 = std::integral_constant<int, 4>;                                              \
-        if ((shape).tw == 32) {                                                                 \
+        if ((shape).tw == 16) {                                                                 \
+            if ((shape).nc == 1) pl_sel(I16{}, N1{});                                           \
+            else pl_sel(I16{}, N2{});                                                           \
+        } else if ((shape).tw == 32) {                                                          \
             if ((shape).nc == 1) pl_sel(I32{}, N1{});                                           \
             else if ((shape).nc == 2) pl_sel(I32{}, N2{});                                      \
             else pl_sel(I32{}, N4{});                                                           \

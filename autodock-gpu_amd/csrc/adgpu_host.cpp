@@ -46,7 +46,7 @@ struct adgpu_context {
     int next_id = 1;
     std::string err;
     std::mutex mu;
-    int team_hint = 0;    // 0 = automatic team width, else 32 or 64 (ADGPU_TEAM_WIDTH)
+    int team_hint = 0;    // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
     // scratch for adgpu_eval_poses
     float* d_geno = nullptr;
     float* d_ene = nullptr;
@@ -302,7 +302,8 @@ const char* adgpu_last_error(adgpu_ctx ctx) { return ctx ? ctx->err.c_str() : "n
 
 int adgpu_set_team_width(adgpu_ctx c, int width) {
     if (!c) return ADGPU_ERR_ARG;
-    if (width != 0 && width != 32 && width != 64) return fail(c, ADGPU_ERR_ARG, "team width must be 0, 32 or 64");
+    if (width != 0 && width != 16 && width != 32 && width != 64)
+        return fail(c, ADGPU_ERR_ARG, "team width must be 0, 16, 32 or 64");
     std::lock_guard<std::mutex> lk(c->mu);
     c->team_hint = width;
     return ADGPU_OK;
@@ -330,7 +331,7 @@ int adgpu_open(int device, adgpu_ctx* out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
     if (const char* tw = std::getenv("ADGPU_TEAM_WIDTH")) {
         int v = std::atoi(tw);
-        c->team_hint = (v == 32 || v == 64) ? v : 0;
+        c->team_hint = (v == 16 || v == 32 || v == 64) ? v : 0;
     }
     *out = c.release();
     return ADGPU_OK;

@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--maps", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--team-width", type=int, default=0, choices=[0, 32, 64])
+    ap.add_argument("--team-width", type=int, default=0, choices=[0, 16, 32, 64])
     # hardware queues of this process, set before the HIP runtime starts (HIP's
     # default is 4): with 8 the library runs the job's runs in four groups on
     # four streams, each on its own queue

@@ -160,7 +160,8 @@ const char* adgpu_last_error(adgpu_ctx ctx);
 /* Version string; also usable without a GPU. */
 const char* adgpu_version(void);
 /* Lanes that evaluate one individual: 0 = chosen per ligand (32 up to 128
- * atoms, 64 above), 32 (two individuals per wavefront) or 64.  Plays the role
+ * atoms, 64 above), 16 (four individuals per wavefront, ligands of at most 32
+ * atoms; wider teams otherwise), 32 (two individuals per wavefront) or 64.  Plays the role
  * of the reference's build-time NUM_OF_THREADS_PER_BLOCK (common/defines.h:29-49)
  * as a run-time choice; results agree within the FP32 tolerance either way. */
 int  adgpu_set_team_width(adgpu_ctx ctx, int width);

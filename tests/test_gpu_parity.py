@@ -183,12 +183,14 @@ def test_million_scale_batch_properties(gpu_ctx):
     helpers.check_energies(case.orc, poses[perm[:400]], e1[perm[:400]])
 
 
-@pytest.mark.parametrize("width", [32, 64])
-def test_team_width_parity(gpu_ctx, width):
-    """Both team widths (two individuals per wavefront, or one) meet the
-    energy bar and the gradient bar against the oracle, for a ligand that
-    needs one atom chunk per lane at width 64 and three at width 32."""
-    case = helpers.synthetic_case(90, 20, 5, 13, npts=64)
+@pytest.mark.parametrize("width,shape", [(32, (90, 20, 5, 13, 64)), (64, (90, 20, 5, 13, 64)),
+                                         (16, (30, 6, 7, 11, 60)), (16, (14, 3, 3, 5, 40))])
+def test_team_width_parity(gpu_ctx, width, shape):
+    """Every team width (four, two or one individuals per wavefront) meets the
+    energy bar and the gradient bar against the oracle: a ligand that needs
+    one atom chunk per lane at width 64 and three at width 32, and ligands of
+    two chunks and one chunk at width 16."""
+    case = helpers.synthetic_case(*shape[:4], npts=shape[4])
     mid, lid = _upload(gpu_ctx, case)
     poses = helpers.poses_for(case, 400, seed=505)
     gpu_ctx.set_team_width(width)

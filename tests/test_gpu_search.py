@@ -27,10 +27,19 @@ def _pair_jobs(ctx, case, sp, seed=(5, 9)):
     return job, oj
 
 
-def test_initpop_and_ga_generation_exact(gpu_ctx, c1g):
+@pytest.mark.parametrize("width", [0, 16])
+def test_initpop_and_ga_generation_exact(gpu_ctx, c1g, width):
     """K1 energies, then K4: selection, crossover and mutation consume the
     same LCG draws, so genes and PRNG states are bit-identical to the oracle;
-    offspring energies meet the energy bar."""
+    offspring energies meet the energy bar (default and 16-lane teams)."""
+    gpu_ctx.set_team_width(width)
+    try:
+        _ga_generations(gpu_ctx, c1g)
+    finally:
+        gpu_ctx.set_team_width(0)
+
+
+def _ga_generations(gpu_ctx, c1g):
     sp = search_params(num_runs=6, pop_size=64, ls="sw", spacing=c1g.spacing)
     job, oj = _pair_jobs(gpu_ctx, c1g, sp)
     job.initpop()
@@ -77,12 +86,22 @@ def test_local_search_schedule_on_flat_energy(gpu_ctx, ls, expected):
     assert list(ev) == [2 + 1 + expected[0] + expected[0]] * 3
 
 
+@pytest.mark.parametrize("width", [0, 16])
 @pytest.mark.parametrize("ls", ["sw", "ad", "adam", "fire"])
-def test_short_local_search_parity(gpu_ctx, c1g, ls):
+def test_short_local_search_parity(gpu_ctx, c1g, ls, width):
     """A few local-search iterations from identical populations: the GPU and
     the oracle take the same discrete decisions for nearly all entities and
-    their continuous state stays within the gradient bar."""
+    their continuous state stays within the gradient bar (default teams and
+    16-lane teams)."""
     sp = search_params(num_runs=4, pop_size=32, ls=ls, max_iters=3, spacing=c1g.spacing)
+    gpu_ctx.set_team_width(width)
+    try:
+        _short_ls(gpu_ctx, c1g, sp)
+    finally:
+        gpu_ctx.set_team_width(0)
+
+
+def _short_ls(gpu_ctx, c1g, sp):
     job, oj = _pair_jobs(gpu_ctx, c1g, sp, seed=(7, 3))
     job.initpop()
     oj.initpop()
@@ -103,7 +122,7 @@ def test_short_local_search_parity(gpu_ctx, c1g, ls):
     d[:, :3] = np.abs(a["pop"][:, :3] - b["pop"][:, :3])
     d[:, 3:] = helpers.angle_diff(a["pop"][:, 3:ng], b["pop"][:, 3:ng])
     close = np.all(d <= np.maximum(1e-2, 1e-3 * np.abs(b["pop"][:, :ng])), axis=1)
-    assert close.mean() >= 0.9, (ls, close.mean())
+    assert close.mean() >= 0.9, (sp.ls_method, close.mean())
     de = np.abs(a["ene"] - b["ene"]) / np.maximum(1.0, np.abs(b["ene"]))
     assert np.mean(de[close] <= 1e-3) >= 0.95
 
