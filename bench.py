@@ -140,9 +140,9 @@ def main():
         return
     os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     if a.gpus != 1:
-        print("bench.py measures one GPU: every GPU box here has one, and the docking "
-              "path shards by ligand/run with no inter-GPU exchange (see the sharding note "
-              "in the design document); run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the docking path shards by ligand or by run with no "
+              "exchange between GPUs during the search, so it runs one process per GPU; "
+              "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import adgpu
     lig, maps, tables, ep = build_inputs()
