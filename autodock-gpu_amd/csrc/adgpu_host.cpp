@@ -28,12 +28,14 @@ struct MapSet {
     GridDev dev{};
     void* data = nullptr;
     size_t bytes = 0;
+    int jobs = 0;         // jobs created on these maps and not yet destroyed
 };
 
 struct LigandSet {
     LigandDev host{};
     void* dev = nullptr;
     int max_map = 0;      // largest atom_map index (must be < the grid's type maps)
+    int jobs = 0;         // jobs created for this ligand and not yet destroyed
 };
 
 }  // namespace
@@ -75,6 +77,7 @@ struct RunGroup {
 struct adgpu_job {
     adgpu_context* ctx = nullptr;
     int map_id = 0, ligand_id = 0;
+    bool counted = false;               // holds a use of its maps and ligand
     SearchDev sd{};
     adgpu_search_params sp{};
     float* pop[2] = {nullptr, nullptr};
@@ -403,6 +406,7 @@ int adgpu_maps_release(adgpu_ctx c, int id) {
     std::lock_guard<std::mutex> lk(c->mu);
     auto it = c->maps.find(id);
     if (it == c->maps.end()) return fail(c, ADGPU_ERR_ARG, "unknown map id");
+    if (it->second.jobs > 0) return fail(c, ADGPU_ERR_STATE, "maps still used by a job (destroy the job first)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(it->second.data);
     c->maps.erase(it);
@@ -438,6 +442,7 @@ int adgpu_ligand_release(adgpu_ctx c, int id) {
     std::lock_guard<std::mutex> lk(c->mu);
     auto it = c->ligands.find(id);
     if (it == c->ligands.end()) return fail(c, ADGPU_ERR_ARG, "unknown ligand id");
+    if (it->second.jobs > 0) return fail(c, ADGPU_ERR_STATE, "ligand still used by a job (destroy the job first)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(it->second.dev);
     c->ligands.erase(it);
@@ -593,6 +598,9 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     HIPCHK(c, hipMemset(j->counters, 0, 16 * ng * sizeof(int32_t)));
     HIPCHK(c, hipMemset(j->d_stop, 0, 16 * sizeof(int32_t)));
     j->h_stop[0] = 0;
+    mi->second.jobs++;
+    li->second.jobs++;
+    j->counted = true;
     *out = j.release();
     return ADGPU_OK;
 }
@@ -600,6 +608,13 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
 int adgpu_job_destroy(adgpu_job_t j) {
     if (!j) return ADGPU_ERR_ARG;
     if (j->ctx) (void)hipSetDevice(j->ctx->device);
+    if (j->ctx && j->counted) {
+        std::lock_guard<std::mutex> lk(j->ctx->mu);
+        auto mi = j->ctx->maps.find(j->map_id);
+        auto li = j->ctx->ligands.find(j->ligand_id);
+        if (mi != j->ctx->maps.end()) mi->second.jobs--;
+        if (li != j->ctx->ligands.end()) li->second.jobs--;
+    }
     for (auto& g : j->groups) {
         if (g.stream) { (void)hipStreamSynchronize(g.stream); (void)hipStreamDestroy(g.stream); }
         for (int r = 0; r < kRing; ++r) {
@@ -624,8 +639,11 @@ int adgpu_job_destroy(adgpu_job_t j) {
     return ADGPU_OK;
 }
 
+// The job's maps and ligand cannot be released while the job exists (their
+// job counts), so the pointers stay valid for the job's lifetime.
 static int job_lookup(adgpu_job_t j, const LigandSet** ls, const MapSet** ms) {
     adgpu_context* c = j->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
     auto mi = c->maps.find(j->map_id);
     auto li = c->ligands.find(j->ligand_id);
     if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_STATE, "map or ligand released");
@@ -841,6 +859,12 @@ static int stopped_generation(adgpu_job_t j, int64_t* gen) {
     return ADGPU_OK;
 }
 
+// Most evals one run can add in one generation: K4 offspring plus every LS
+// slot at its iteration cap (two evaluations per Solis-Wets iteration).
+static int64_t max_evals_per_generation(const adgpu_job* j) {
+    return (int64_t)(j->sp.pop_size - 1) + (int64_t)j->sd.num_ls * 2 * std::max<int64_t>(1, j->sp.max_num_of_iters);
+}
+
 int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens, int64_t* out_total) {
     if (!j || max_new < 0) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
@@ -866,8 +890,7 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
     }
     // most evals one run can add in one generation: K4 offspring plus every LS
     // slot at its iteration cap (two evaluations per Solis-Wets iteration)
-    const int64_t emax = (int64_t)(j->sp.pop_size - 1) +
-                         (int64_t)j->sd.num_ls * 2 * std::max<int64_t>(1, j->sp.max_num_of_iters);
+    const int64_t emax = max_evals_per_generation(j);
     auto bound = [&](int64_t cg) {   // upper bound of the total after cg generations
         int64_t t = 0;
         for (auto& g : j->groups) {

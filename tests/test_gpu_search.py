@@ -211,3 +211,22 @@ def test_results_do_not_depend_on_launch_shape(gpu_ctx, c1g, ls, tmp_path):
     for o in outs[1:]:
         for k in ("pop", "ene", "ev"):
             assert np.array_equal(o[k], outs[0][k]), k
+
+
+def test_maps_and_ligand_outlive_their_jobs(gpu_ctx):
+    """A map set or ligand that a job uses cannot be released until the job
+    is destroyed (the job keeps device pointers to both)."""
+    case = helpers.synthetic_case(12, 2, 3, 5, npts=40)
+    mid = gpu_ctx.maps_upload(case.packed, case.size, case.maps.num_map_atypes, case.spacing)
+    lid = gpu_ctx.ligand_upload(case.tables, case.ep)
+    sp = search_params(num_runs=2, pop_size=10, max_evals=2000, ls="sw", spacing=case.spacing)
+    init = O.initial_population((1,), 2, 10, case.maps.size_angstr, case.spacing)
+    job = gpu_ctx.job(mid, lid, sp, init, O.cmwc_uints((1,), 2 * 10 * 64))
+    for release, ident in ((gpu_ctx.maps_release, mid), (gpu_ctx.ligand_release, lid)):
+        with pytest.raises(adgpu.AdgpuError):
+            release(ident)
+    job.initpop()
+    job.run()
+    job.destroy()
+    gpu_ctx.ligand_release(lid)
+    gpu_ctx.maps_release(mid)
