@@ -14,7 +14,7 @@ from . import _abi
 from ._abi import (EVAL_ENERGRAD, EVAL_ENERGY, EVAL_GRADIENT, GENOTYPE_STRIDE, LS_AD, LS_ADAM, LS_FIRE, LS_NONE,
                    LS_SW, MAP_BF16, MAP_F16, MAP_F32, NWI, CGrid, CLigand, energy_params, search_params)
 
-__all__ = ["Context", "Job", "AdgpuError", "energy_params", "search_params", "MAP_F32", "MAP_F16", "MAP_BF16",
+__all__ = ["Context", "Job", "Server", "AdgpuError", "energy_params", "search_params", "MAP_F32", "MAP_F16", "MAP_BF16",
            "EVAL_ENERGY", "EVAL_ENERGRAD", "EVAL_GRADIENT", "LS_NONE", "LS_SW", "LS_AD", "LS_ADAM", "LS_FIRE",
            "GENOTYPE_STRIDE", "NWI"]
 
@@ -238,3 +238,105 @@ class Job:
         r = _abi.AutostopRecord()
         self._c(self.lib.adgpu_job_autostop_final(self.h, C.byref(r)), "adgpu_job_autostop_final")
         return r.as_dict()
+
+
+class Server:
+    """Serving loop (adgpu_server_*): docking requests run concurrently in
+    `slots` slots, admitted in arrival order when a slot is free and their
+    device memory fits `memory_budget` bytes (0 = 90 % of free memory)."""
+
+    def __init__(self, ctx: Context, slots: int = 8, run_groups: int = 0, memory_budget: int = 0):
+        self.ctx, self.lib = ctx, ctx.lib
+        cfg = _abi.ServerConfig(int(slots), int(run_groups), int(memory_budget))
+        h = C.c_void_p()
+        ctx._check(self.lib.adgpu_server_create(ctx.h, C.byref(cfg), C.byref(h)), "adgpu_server_create")
+        self.h = h
+        self._shapes = {}
+
+    def submit(self, map_id, tables, sp, init_pop, seeds, ep=None, autostop=None, keep_population=False,
+               rescore_f64=False) -> int:
+        """Queues one docking; autostop = (stopstd, as_frequency, min_as_evals)
+        or None.  Returns the ticket."""
+        cl = CLigand(tables)
+        ep = ep or energy_params()
+        ne = sp.num_runs * sp.pop_size
+        ip = _f32(init_pop).reshape(-1)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1)
+        if ip.size != ne * GENOTYPE_STRIDE or sd.size != ne * NWI:
+            raise ValueError("init_pop / seeds sizes do not match num_runs * pop_size")
+        q = _abi.Request()
+        q.map_id = int(map_id)
+        q.ligand = C.pointer(cl.s)
+        q.energy = C.pointer(ep)
+        q.search = C.pointer(sp)
+        q.init_population = ip.ctypes.data_as(_fp)
+        q.prng_seeds = sd.ctypes.data_as(_up)
+        if autostop is not None:
+            q.autostop = 1
+            q.stopstd, q.as_frequency, q.min_as_evals = float(autostop[0]), int(autostop[1]), int(autostop[2])
+        q.flags = (_abi.REQ_KEEP_POPULATION if keep_population else 0) | (_abi.REQ_RESCORE_F64 if rescore_f64 else 0)
+        t = C.c_int64()
+        self.ctx._check(self.lib.adgpu_server_submit(self.h, C.byref(q), C.byref(t)), "adgpu_server_submit")
+        self._shapes[t.value] = (sp.num_runs, sp.pop_size, bool(keep_population), bool(rescore_f64))
+        return t.value
+
+    def next_done(self, timeout_ms: int = -1):
+        """Ticket of the oldest finished request not returned yet, or None
+        after timeout_ms."""
+        t = C.c_int64()
+        rc = self.lib.adgpu_server_next_done(self.h, int(timeout_ms), C.byref(t))
+        if rc == _abi.ERR_TIMEOUT:
+            return None
+        self.ctx._check(rc, "adgpu_server_next_done")
+        return t.value
+
+    def result(self, ticket: int, timeout_ms: int = -1):
+        """Waits for the request and returns its result (and forgets it):
+        a dict with status, timing (ms on the server clock), generations,
+        total_evals, best_genotypes, best_energies, and best_f64 /
+        population / energies when requested at submit."""
+        nr, pz, keep, f64 = self._shapes[ticket]
+        r = _abi.RequestResult()
+        geno = np.zeros((nr, GENOTYPE_STRIDE), np.float32)
+        e = np.zeros(nr, np.float32)
+        b64 = np.zeros((nr, 3)) if f64 else None
+        pop = np.zeros((nr * pz, GENOTYPE_STRIDE), np.float32) if keep else None
+        ene = np.zeros(nr * pz, np.float32) if keep else None
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.adgpu_server_result(self.h, int(ticket), int(timeout_ms), C.byref(r), geno.ctypes.data_as(_fp),
+                                          e.ctypes.data_as(_fp), b64.ctypes.data_as(dp) if f64 else None,
+                                          pop.ctypes.data_as(_fp) if keep else None,
+                                          ene.ctypes.data_as(_fp) if keep else None)
+        if rc == _abi.ERR_TIMEOUT:
+            return None
+        self.ctx._check(rc, "adgpu_server_result")
+        del self._shapes[ticket]
+        d = r.as_dict()
+        d.update(best_genotypes=geno, best_energies=e)
+        if f64:
+            d["best_f64"] = b64
+        if keep:
+            d.update(population=pop, energies=ene)
+        return d
+
+    def stats(self):
+        s = _abi.ServerStats()
+        self.ctx._check(self.lib.adgpu_server_stats_get(self.h, C.byref(s)), "adgpu_server_stats_get")
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adgpu_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -70,6 +70,44 @@ class AutostopRecord(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("run_groups", C.c_int32), ("memory_budget", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("map_id", C.c_int32), ("ligand", C.POINTER(Ligand)), ("energy", C.POINTER(EnergyParams)),
+                ("search", C.POINTER(SearchParams)), ("init_population", _fp), ("prng_seeds", _up),
+                ("autostop", C.c_int32), ("as_frequency", C.c_int32), ("stopstd", C.c_float),
+                ("min_as_evals", C.c_int64), ("flags", C.c_int32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("slot", C.c_int32), ("num_runs", C.c_int32), ("pop_size", C.c_int32),
+                ("generations", C.c_int64), ("total_evals", C.c_int64), ("device_bytes", C.c_int64),
+                ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double),
+                ("message", C.c_char * 160)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["message"] = self.message.decode(errors="replace")
+        return d
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("queued", C.c_int64), ("active", C.c_int64), ("peak_active", C.c_int64),
+                ("bytes_in_use", C.c_int64), ("peak_bytes", C.c_int64), ("memory_budget", C.c_int64),
+                ("slot_reallocations", C.c_int64), ("now_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_KEEP_POPULATION = 1
+REQ_RESCORE_F64 = 2
+ERR_TIMEOUT = 6
+
+
 def _ptr(a, ctype):
     return a.ctypes.data_as(ctype)
 
@@ -195,6 +233,12 @@ def load_library(path: str | None = None):
     lib.adgpu_job_autostop_check.argtypes = [vp, rp]
     lib.adgpu_job_autostop_log.argtypes = [vp, C.c_int, C.c_int, rp, C.POINTER(C.c_int)]
     lib.adgpu_job_autostop_final.argtypes = [vp, rp]
+    lib.adgpu_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+    lib.adgpu_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+    lib.adgpu_server_next_done.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    lib.adgpu_server_result.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(RequestResult), _fp, _fp, _dp, _fp, _fp]
+    lib.adgpu_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+    lib.adgpu_server_destroy.argtypes = [vp]
     if path is None:
         _LIB = lib
     return lib
@@ -205,4 +249,6 @@ EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "a
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",
             "adgpu_job_kernel_ls", "adgpu_job_kernel_sum_and_swap", "adgpu_job_generation", "adgpu_job_run",
             "adgpu_job_download", "adgpu_job_upload", "adgpu_job_kernel_time", "adgpu_job_kernel_busy", "adgpu_job_set_autostop",
-            "adgpu_job_autostop_check", "adgpu_job_autostop_log", "adgpu_job_autostop_final"]
+            "adgpu_job_autostop_check", "adgpu_job_autostop_log", "adgpu_job_autostop_final",
+            "adgpu_server_create", "adgpu_server_submit", "adgpu_server_next_done", "adgpu_server_result",
+            "adgpu_server_stats_get", "adgpu_server_destroy"]

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +20,7 @@
 #include <vector>
 
 #include "adgpu_internal.h"
+#include "adgpu_slots.h"
 
 using namespace adgpu;
 
@@ -46,9 +48,8 @@ struct adgpu_context {
     std::map<int, MapSet> maps;
     std::map<int, LigandSet> ligands;
     int next_id = 1;
-    std::string err;
     std::mutex mu;
-    int team_hint = 0;    // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
+    std::atomic<int> team_hint{0};   // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
     // scratch for adgpu_eval_poses
     float* d_geno = nullptr;
     float* d_ene = nullptr;
@@ -74,10 +75,27 @@ struct RunGroup {
     hipEvent_t ev_done[kRing] = {}, ev_gen[2][kRing] = {}, ev_ls[2][kRing] = {};
 };
 
+constexpr int kMaxGroups = 8;
+
 struct adgpu_job {
     adgpu_context* ctx = nullptr;
     int map_id = 0, ligand_id = 0;
     bool counted = false;               // holds a use of its maps and ligand
+    // capacities of the device buffers; a serving slot reuses them for the
+    // next request and grows them only when a request needs more
+    size_t cap_ne = 0, cap_ls = 0;
+    int cap_runs = 0;
+    size_t as_bins_cap = 0;
+    int as_log_alloc = 0;
+    int64_t reallocs = 0;               // buffer growths after the first allocation
+    // serving slot: the ligand block belongs to the job (not a context id)
+    bool own_lig = false;
+    int want_groups = 1;                // run groups of a serving slot (fewer when a request has fewer runs)
+    LigandSet own{};
+    size_t own_cap = 0;
+    float* d_best = nullptr;            // per-run best genotypes for FP64 re-scoring
+    double* d_f64 = nullptr;
+    int best_cap = 0;
     SearchDev sd{};
     adgpu_search_params sp{};
     float* pop[2] = {nullptr, nullptr};
@@ -121,12 +139,17 @@ LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, 
     la.lig_dev = reinterpret_cast<const LigandDev*>(ls.dev);
     la.lig_host = &ls.host;
     la.grid = ms.dev;
-    la.team_hint = c->team_hint;
+    la.team_hint = c->team_hint.load(std::memory_order_relaxed);
     return la;
 }
 
+// Message of the calling thread's last failed call (jobs of one context may
+// be driven from several threads, e.g. the serving loop's workers).
+thread_local std::string t_err;
+
 int fail(adgpu_context* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
+    (void)c;
+    t_err = msg;
     return code;
 }
 
@@ -301,7 +324,7 @@ extern "C" {
 
 const char* adgpu_version(void) { return "adgpu-mi355x 0.1 (gfx950)"; }
 
-const char* adgpu_last_error(adgpu_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* adgpu_last_error(adgpu_ctx ctx) { return ctx ? t_err.c_str() : "null context"; }
 
 int adgpu_set_team_width(adgpu_ctx c, int width) {
     if (!c) return ADGPU_ERR_ARG;
@@ -415,21 +438,16 @@ int adgpu_maps_release(adgpu_ctx c, int id) {
 
 int adgpu_ligand_upload(adgpu_ctx c, const adgpu_ligand* L, const adgpu_energy_params* ep, int* out_id) {
     if (!c || !L || !ep || !out_id) return fail(c, ADGPU_ERR_ARG, "null argument");
-    std::vector<char> blob;
-    LigandDev hd;
-    int rc = build_ligand(c, L, ep, blob, hd);
+    PreparedLigand pl;
+    int rc = prepare_ligand(c, L, ep, pl);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(c, hipSetDevice(c->device));
     LigandSet ls;
-    ls.host = hd;
-    for (int a = 0; a < L->num_atoms; ++a)
-        if (!(L->ignore_inter && L->ignore_inter[a]) && L->atom_map[a] > ls.max_map) ls.max_map = L->atom_map[a];
-    for (int a = 0; a < L->num_atoms; ++a)
-        if (L->atom_map[a] < 0 && !(L->ignore_inter && L->ignore_inter[a]))
-            return fail(c, ADGPU_ERR_ARG, "negative atom_map index");
-    if (hipMalloc(&ls.dev, blob.size()) != hipSuccess) return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(ligand) failed");
-    hipError_t e = hipMemcpy(ls.dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    ls.host = pl.hd;
+    ls.max_map = pl.max_map;
+    if (hipMalloc(&ls.dev, pl.blob.size()) != hipSuccess) return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(ligand) failed");
+    hipError_t e = hipMemcpy(ls.dev, pl.blob.data(), pl.blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { (void)hipFree(ls.dev); return fail(c, ADGPU_ERR_HIP, hipGetErrorString(e)); }
     int id = c->next_id++;
     c->ligands[id] = ls;
@@ -523,6 +541,145 @@ int adgpu_rescore_f64(adgpu_ctx c, int map_id, int lig_id, const float* genotype
     return ADGPU_OK;
 }
 
+// Run groups: streams and their per-generation events.
+static bool make_group(RunGroup& g) {
+    bool ok = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) == hipSuccess;
+    for (int r = 0; r < kRing && ok; ++r)
+        ok = hipEventCreateWithFlags(&g.ev_done[r], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&g.ev_gen[0][r]) == hipSuccess && hipEventCreate(&g.ev_gen[1][r]) == hipSuccess &&
+             hipEventCreate(&g.ev_ls[0][r]) == hipSuccess && hipEventCreate(&g.ev_ls[1][r]) == hipSuccess;
+    return ok;
+}
+
+static void destroy_group(RunGroup& g) {
+    if (g.stream) { (void)hipStreamSynchronize(g.stream); (void)hipStreamDestroy(g.stream); }
+    for (int r = 0; r < kRing; ++r) {
+        if (g.ev_done[r]) (void)hipEventDestroy(g.ev_done[r]);
+        for (int h = 0; h < 2; ++h) {
+            if (g.ev_gen[h][r]) (void)hipEventDestroy(g.ev_gen[h][r]);
+            if (g.ev_ls[h][r]) (void)hipEventDestroy(g.ev_ls[h][r]);
+        }
+    }
+    g = RunGroup{};
+}
+
+// Sets the number of run groups (at most one per run) and splits the runs
+// evenly over them.
+static int job_set_groups(adgpu_job* j, int ng, int nrun) {
+    ng = std::max(1, std::min(ng, std::min(nrun, kMaxGroups)));
+    while ((int)j->groups.size() > ng) {
+        destroy_group(j->groups.back());
+        j->groups.pop_back();
+    }
+    while ((int)j->groups.size() < ng) {
+        j->groups.emplace_back();
+        if (!make_group(j->groups.back())) return fail(j->ctx, ADGPU_ERR_NOMEM, "stream or event creation failed");
+    }
+    for (int k = 0; k < ng; ++k) {
+        j->groups[k].r0 = (int)((int64_t)nrun * k / ng);
+        j->groups[k].r1 = (int)((int64_t)nrun * (k + 1) / ng);
+    }
+    return ADGPU_OK;
+}
+
+// Allocations of a job that do not depend on the request's size.
+static int job_open(adgpu_job* j) {
+    bool ok = hipMalloc(&j->counters, 16 * kMaxGroups * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&j->d_stop, 16 * sizeof(int32_t)) == hipSuccess &&
+              hipHostMalloc(&j->h_stop, 16 * sizeof(int32_t)) == hipSuccess &&
+              hipEventCreateWithFlags(&j->ev_test, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreate(&j->ev_origin) == hipSuccess;
+    return ok ? ADGPU_OK : fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
+}
+
+// Grows the population, energy, PRNG and FIRE buffers to hold nrun runs of
+// pop individuals with nls local-search slots per run.  Buffers that already
+// fit are kept; growing frees the old ones (hipFree synchronizes the device).
+static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire) {
+    const size_t ne = (size_t)nrun * pop;
+    if (ne > j->cap_ne) {
+        if (j->cap_ne) j->reallocs++;
+        (void)hipFree(j->pop[0]); (void)hipFree(j->pop[1]);
+        (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
+        (void)hipFree(j->evals_new); (void)hipFree(j->prng);
+        j->pop[0] = j->pop[1] = j->ene[0] = j->ene[1] = nullptr;
+        j->evals_new = nullptr;
+        j->prng = nullptr;
+        j->cap_ne = 0;
+        bool ok = hipMalloc(&j->pop[0], ne * kGenes * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&j->pop[1], ne * kGenes * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&j->ene[0], ne * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&j->ene[1], ne * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&j->evals_new, ne * sizeof(int32_t)) == hipSuccess &&
+                  hipMalloc(&j->prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess;
+        if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
+        j->cap_ne = ne;
+    }
+    if (nrun > j->cap_runs) {
+        if (j->cap_runs) j->reallocs++;
+        (void)hipFree(j->evals_runs);
+        if (j->h_ring) (void)hipHostFree(j->h_ring);
+        j->evals_runs = nullptr;
+        j->h_ring = nullptr;
+        j->cap_runs = 0;
+        bool ok = hipMalloc(&j->evals_runs, nrun * sizeof(int32_t)) == hipSuccess &&
+                  hipHostMalloc(&j->h_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess;
+        if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
+        j->cap_runs = nrun;
+    }
+    const size_t nls_all = (size_t)nrun * (nls > 0 ? nls : 1);
+    if (fire && nls_all > j->cap_ls) {
+        if (j->cap_ls) j->reallocs++;
+        (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
+        j->ls_ent = nullptr;
+        j->ls_grad = nullptr;
+        j->cap_ls = 0;
+        bool ok = hipMalloc(&j->ls_ent, nls_all * sizeof(int32_t)) == hipSuccess &&
+                  hipMalloc(&j->ls_grad, nls_all * kGenes * sizeof(float)) == hipSuccess;
+        if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
+        j->cap_ls = nls_all;
+    }
+    return ADGPU_OK;
+}
+
+// Starts a search: uploads the initial population and PRNG seeds, clears
+// energies, eval counters, work queues and the stop flag, and resets the
+// generation state (the job's groups and buffers are already sized).
+static int job_start(adgpu_job* j, const adgpu_search_params* sp, int num_genes, const float* init_pop,
+                     const uint32_t* seeds) {
+    adgpu_context* c = j->ctx;
+    j->sp = *sp;
+    j->sd = to_dev(sp, num_genes);
+    const int nrun = sp->num_runs;
+    const size_t ne = (size_t)nrun * sp->pop_size;
+    for (auto& g : j->groups) {
+        g.parity = g.parity0 = 0;
+        g.done = g.enq = 0;
+        g.totals.clear();
+    }
+    j->generation = 0;
+    j->stopped = false;
+    j->as_enabled = false;
+    j->origin_set = false;
+    j->spans[0].clear();
+    j->spans[1].clear();
+    j->kernel_ms[0] = j->kernel_ms[1] = 0.0;
+    j->kernel_n[0] = j->kernel_n[1] = 0;
+    hipStream_t s = j->groups[0].stream;
+    HIPCHK(c, hipMemcpyAsync(j->pop[0], init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(j->pop[1], init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(j->prng, seeds, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(j->evals_runs, 0, nrun * sizeof(int32_t), s));
+    HIPCHK(c, hipMemsetAsync(j->evals_new, 0, ne * sizeof(int32_t), s));
+    HIPCHK(c, hipMemsetAsync(j->ene[0], 0, ne * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(j->ene[1], 0, ne * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(j->counters, 0, 16 * kMaxGroups * sizeof(int32_t), s));
+    HIPCHK(c, hipMemsetAsync(j->d_stop, 0, 16 * sizeof(int32_t), s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    j->h_stop[0] = 0;
+    return ADGPU_OK;
+}
+
 int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_params* sp, const float* init_pop,
                      const uint32_t* seeds, adgpu_job_t* out) {
     if (!c || !sp || !init_pop || !seeds || !out) return fail(c, ADGPU_ERR_ARG, "null argument");
@@ -535,16 +692,14 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
         return fail(c, ADGPU_ERR_ARG, "ligand atom_map index exceeds the grid's type maps");
     int rc = check_search(c, sp, li->second.host.num_genes);
     if (rc) return rc;
+    const int num_genes = li->second.host.num_genes;
+    if (to_dev(sp, num_genes).num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
     HIPCHK(c, hipSetDevice(c->device));
     std::unique_ptr<adgpu_job> j(new adgpu_job());
     j->ctx = c;
     j->map_id = map_id;
     j->ligand_id = lig_id;
-    j->sp = *sp;
-    j->sd = to_dev(sp, li->second.host.num_genes);
-    if (j->sd.num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
     const int nrun = sp->num_runs;
-    const size_t ne = (size_t)nrun * sp->pop_size;
     // run groups: ADGPU_RUN_GROUPS streams.  Groups whose streams share a
     // hardware queue serialize, so the default follows the HIP runtime's
     // hardware queues per process (GPU_MAX_HW_QUEUES, 4 unless set): 2 groups
@@ -554,50 +709,15 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     if (const char* q = std::getenv("GPU_MAX_HW_QUEUES"))
         if (std::atoi(q) >= 6) ng = 4;
     if (const char* e = std::getenv("ADGPU_RUN_GROUPS")) ng = std::atoi(e);
-    ng = std::max(1, std::min(ng, std::min(nrun, 8)));
-    bool ok = hipMalloc(&j->pop[0], ne * kGenes * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->pop[1], ne * kGenes * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->ene[0], ne * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->ene[1], ne * sizeof(float)) == hipSuccess &&
-              hipMalloc(&j->evals_new, ne * sizeof(int32_t)) == hipSuccess &&
-              hipMalloc(&j->evals_runs, nrun * sizeof(int32_t)) == hipSuccess &&
-              hipMalloc(&j->prng, ne * ADGPU_NWI * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&j->counters, 16 * ng * sizeof(int32_t)) == hipSuccess &&
-              hipMalloc(&j->d_stop, 16 * sizeof(int32_t)) == hipSuccess &&
-              hipHostMalloc(&j->h_stop, 16 * sizeof(int32_t)) == hipSuccess &&
-              hipEventCreateWithFlags(&j->ev_test, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreate(&j->ev_origin) == hipSuccess &&
-              hipHostMalloc(&j->h_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess;
-    if (ok && sp->ls_method == ADGPU_LS_FIRE) {
-        const size_t nls = (size_t)nrun * (j->sd.num_ls > 0 ? j->sd.num_ls : 1);
-        ok = hipMalloc(&j->ls_ent, nls * sizeof(int32_t)) == hipSuccess &&
-             hipMalloc(&j->ls_grad, nls * kGenes * sizeof(float)) == hipSuccess;
-    }
-    j->groups.resize(ng);
-    for (int k = 0; k < ng && ok; ++k) {
-        RunGroup& g = j->groups[k];
-        g.r0 = (int)((int64_t)nrun * k / ng);
-        g.r1 = (int)((int64_t)nrun * (k + 1) / ng);
-        ok = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) == hipSuccess;
-        for (int r = 0; r < kRing && ok; ++r)
-            ok = hipEventCreateWithFlags(&g.ev_done[r], hipEventDisableTiming) == hipSuccess &&
-                 hipEventCreate(&g.ev_gen[0][r]) == hipSuccess && hipEventCreate(&g.ev_gen[1][r]) == hipSuccess &&
-                 hipEventCreate(&g.ev_ls[0][r]) == hipSuccess && hipEventCreate(&g.ev_ls[1][r]) == hipSuccess;
-    }
-    if (!ok) {
+    rc = job_open(j.get());
+    if (!rc) rc = job_set_groups(j.get(), ng, nrun);
+    if (!rc) rc = job_reserve(j.get(), nrun, sp->pop_size, to_dev(sp, num_genes).num_ls, sp->ls_method == ADGPU_LS_FIRE);
+    if (!rc) rc = job_start(j.get(), sp, num_genes, init_pop, seeds);
+    if (rc) {
+        const std::string msg = t_err;
         adgpu_job_destroy(j.release());
-        return fail(c, ADGPU_ERR_NOMEM, "job allocation failed");
+        return fail(c, rc, msg);
     }
-    HIPCHK(c, hipMemcpy(j->pop[0], init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(j->pop[1], init_pop, ne * kGenes * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(j->prng, seeds, ne * ADGPU_NWI * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(j->evals_runs, 0, nrun * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(j->evals_new, 0, ne * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(j->ene[0], 0, ne * sizeof(float)));
-    HIPCHK(c, hipMemset(j->ene[1], 0, ne * sizeof(float)));
-    HIPCHK(c, hipMemset(j->counters, 0, 16 * ng * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(j->d_stop, 0, 16 * sizeof(int32_t)));
-    j->h_stop[0] = 0;
     mi->second.jobs++;
     li->second.jobs++;
     j->counted = true;
@@ -613,24 +733,16 @@ int adgpu_job_destroy(adgpu_job_t j) {
         auto mi = j->ctx->maps.find(j->map_id);
         auto li = j->ctx->ligands.find(j->ligand_id);
         if (mi != j->ctx->maps.end()) mi->second.jobs--;
-        if (li != j->ctx->ligands.end()) li->second.jobs--;
+        if (!j->own_lig && li != j->ctx->ligands.end()) li->second.jobs--;
     }
-    for (auto& g : j->groups) {
-        if (g.stream) { (void)hipStreamSynchronize(g.stream); (void)hipStreamDestroy(g.stream); }
-        for (int r = 0; r < kRing; ++r) {
-            if (g.ev_done[r]) (void)hipEventDestroy(g.ev_done[r]);
-            for (int h = 0; h < 2; ++h) {
-                if (g.ev_gen[h][r]) (void)hipEventDestroy(g.ev_gen[h][r]);
-                if (g.ev_ls[h][r]) (void)hipEventDestroy(g.ev_ls[h][r]);
-            }
-        }
-    }
+    for (auto& g : j->groups) destroy_group(g);
     (void)hipFree(j->pop[0]); (void)hipFree(j->pop[1]);
     (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
     (void)hipFree(j->evals_new); (void)hipFree(j->evals_runs); (void)hipFree(j->prng);
     (void)hipFree(j->counters); (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
     (void)hipFree(j->d_stop); (void)hipFree(j->d_as); (void)hipFree(j->d_as_bins);
     (void)hipFree(j->d_as_log); (void)hipFree(j->d_as_out);
+    (void)hipFree(j->own.dev); (void)hipFree(j->d_best); (void)hipFree(j->d_f64);
     if (j->ev_test) (void)hipEventDestroy(j->ev_test);
     if (j->ev_origin) (void)hipEventDestroy(j->ev_origin);
     if (j->h_stop) (void)hipHostFree(j->h_stop);
@@ -645,10 +757,16 @@ static int job_lookup(adgpu_job_t j, const LigandSet** ls, const MapSet** ms) {
     adgpu_context* c = j->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     auto mi = c->maps.find(j->map_id);
-    auto li = c->ligands.find(j->ligand_id);
-    if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_STATE, "map or ligand released");
-    *ls = &li->second;
+    if (mi == c->maps.end()) return fail(c, ADGPU_ERR_STATE, "map set released");
     *ms = &mi->second;
+    if (j->own_lig) {   // a serving slot's own ligand block
+        if (!j->counted) return fail(c, ADGPU_ERR_STATE, "serving slot not bound to a request");
+        *ls = &j->own;
+        return ADGPU_OK;
+    }
+    auto li = c->ligands.find(j->ligand_id);
+    if (li == c->ligands.end()) return fail(c, ADGPU_ERR_STATE, "ligand released");
+    *ls = &li->second;
     return ADGPU_OK;
 }
 
@@ -998,13 +1116,25 @@ int adgpu_job_set_autostop(adgpu_job_t j, int enable, float stopstd, int as_freq
     const size_t nb = 3 * ((size_t)pop + 1);
     if (!j->d_as) {
         if (hipMalloc(&j->d_as, sizeof(AutoStopDev)) != hipSuccess ||
-            hipMalloc(&j->d_as_bins, nb * sizeof(float)) != hipSuccess ||
             hipMalloc(&j->d_as_out, sizeof(adgpu_autostop_record)) != hipSuccess)
             return fail(c, ADGPU_ERR_NOMEM, "AutoStop allocation failed");
     }
-    if (j->d_as_log && j->as_p.log_cap != p.log_cap) { (void)hipFree(j->d_as_log); j->d_as_log = nullptr; }
-    if (!j->d_as_log && hipMalloc(&j->d_as_log, (size_t)p.log_cap * sizeof(adgpu_autostop_record)) != hipSuccess)
-        return fail(c, ADGPU_ERR_NOMEM, "AutoStop allocation failed");
+    if (nb > j->as_bins_cap) {   // kept when large enough (a serving slot's next request)
+        (void)hipFree(j->d_as_bins);
+        j->d_as_bins = nullptr;
+        j->as_bins_cap = 0;
+        if (hipMalloc(&j->d_as_bins, nb * sizeof(float)) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "AutoStop allocation failed");
+        j->as_bins_cap = nb;
+    }
+    if (p.log_cap > j->as_log_alloc) {
+        (void)hipFree(j->d_as_log);
+        j->d_as_log = nullptr;
+        j->as_log_alloc = 0;
+        if (hipMalloc(&j->d_as_log, (size_t)p.log_cap * sizeof(adgpu_autostop_record)) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "AutoStop allocation failed");
+        j->as_log_alloc = p.log_cap;
+    }
     // the AutoStop constructor, autostop.hpp:129-146
     AutoStopDev st{};
     st.first_time = 1;
@@ -1171,3 +1301,165 @@ int adgpu_job_kernel_time(adgpu_job_t j, int cls, double* avg_ms, int64_t* launc
 }
 
 }  // extern "C"
+
+// ---- serving slots (adgpu_slots.h) ------------------------------------------
+
+namespace adgpu {
+
+int set_error(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+const char* thread_error() { return t_err.c_str(); }
+int context_device(const adgpu_context* c) { return c->device; }
+int64_t slot_reallocations(const adgpu_job* j) { return j->reallocs; }
+
+int prepare_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_params* ep, PreparedLigand& out) {
+    if (!L || !ep) return fail(c, ADGPU_ERR_ARG, "null argument");
+    int rc = build_ligand(c, L, ep, out.blob, out.hd);
+    if (rc) return rc;
+    out.max_map = 0;
+    for (int a = 0; a < L->num_atoms; ++a) {
+        if (L->ignore_inter && L->ignore_inter[a]) continue;
+        if (L->atom_map[a] < 0) return fail(c, ADGPU_ERR_ARG, "negative atom_map index");
+        out.max_map = std::max(out.max_map, (int)L->atom_map[a]);
+    }
+    return ADGPU_OK;
+}
+
+int check_request(adgpu_context* c, int map_id, const PreparedLigand& lig, const adgpu_search_params* sp) {
+    if (!sp) return fail(c, ADGPU_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto mi = c->maps.find(map_id);
+    if (mi == c->maps.end()) return fail(c, ADGPU_ERR_ARG, "unknown map id");
+    if (lig.max_map >= mi->second.dev.nmaps - 2)
+        return fail(c, ADGPU_ERR_ARG, "ligand atom_map index exceeds the grid's type maps");
+    int rc = check_search(c, sp, lig.hd.num_genes);
+    if (rc) return rc;
+    if (to_dev(sp, lig.hd.num_genes).num_ls > sp->pop_size) return fail(c, ADGPU_ERR_ARG, "lsearch_rate above 100 %");
+    return ADGPU_OK;
+}
+
+size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params* sp, int as_frequency) {
+    const size_t nrun = (size_t)sp->num_runs, ne = nrun * (size_t)sp->pop_size;
+    size_t b = 2 * ne * kGenes * sizeof(float) + 2 * ne * sizeof(float) + ne * sizeof(int32_t) +
+               ne * ADGPU_NWI * sizeof(uint32_t) + nrun * sizeof(int32_t) + lig.blob.size() +
+               nrun * (kGenes * sizeof(float) + 3 * sizeof(double));   // re-scoring scratch
+    if (sp->ls_method == ADGPU_LS_FIRE) {
+        const size_t nls = (size_t)std::max(1, to_dev(sp, lig.hd.num_genes).num_ls);
+        b += nrun * nls * (sizeof(int32_t) + kGenes * sizeof(float));
+    }
+    if (as_frequency > 0) {   // log_cap as adgpu_job_set_autostop sizes it
+        const int64_t tests = sp->max_generations / as_frequency + 2;
+        const size_t cap = (size_t)std::max<int64_t>(16, std::min<int64_t>(tests, 16384));
+        b += sizeof(AutoStopDev) + 3 * ((size_t)sp->pop_size + 1) * sizeof(float) +
+             (cap + 1) * sizeof(adgpu_autostop_record);
+    }
+    return b;
+}
+
+int slot_open(adgpu_context* c, int run_groups, adgpu_job** out) {
+    *out = nullptr;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::unique_ptr<adgpu_job> j(new adgpu_job());
+    j->ctx = c;
+    j->own_lig = true;
+    j->want_groups = std::max(1, std::min(run_groups, kMaxGroups));
+    int rc = job_open(j.get());
+    if (!rc) rc = job_set_groups(j.get(), run_groups, run_groups);
+    if (rc) {
+        const std::string msg = t_err;
+        adgpu_job_destroy(j.release());
+        return fail(c, rc, msg);
+    }
+    *out = j.release();
+    return ADGPU_OK;
+}
+
+int slot_bind(adgpu_job* j, int map_id, const PreparedLigand& lig, const adgpu_search_params* sp,
+              const float* init_pop, const uint32_t* seeds) {
+    adgpu_context* c = j->ctx;
+    if (!j->own_lig || j->counted) return fail(c, ADGPU_ERR_STATE, "serving slot already bound");
+    HIPCHK(c, hipSetDevice(c->device));
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto mi = c->maps.find(map_id);
+        if (mi == c->maps.end()) return fail(c, ADGPU_ERR_ARG, "unknown map id");
+        mi->second.jobs++;
+        j->map_id = map_id;
+        j->counted = true;
+    }
+    const int nrun = sp->num_runs;
+    const SearchDev sd = to_dev(sp, lig.hd.num_genes);
+    int rc = job_set_groups(j, j->want_groups, nrun);
+    if (!rc) rc = job_reserve(j, nrun, sp->pop_size, sd.num_ls, sp->ls_method == ADGPU_LS_FIRE);
+    if (!rc && lig.blob.size() > j->own_cap) {
+        if (j->own_cap) j->reallocs++;
+        (void)hipFree(j->own.dev);
+        j->own.dev = nullptr;
+        j->own_cap = 0;
+        const size_t cap = (lig.blob.size() + 65535) & ~size_t(65535);
+        if (hipMalloc(&j->own.dev, cap) != hipSuccess) rc = fail(c, ADGPU_ERR_NOMEM, "hipMalloc(ligand) failed");
+        else j->own_cap = cap;
+    }
+    if (!rc) {
+        j->own.host = lig.hd;
+        j->own.max_map = lig.max_map;
+        const hipError_t e = hipMemcpyAsync(j->own.dev, lig.blob.data(), lig.blob.size(), hipMemcpyHostToDevice,
+                                            j->groups[0].stream);
+        if (e != hipSuccess) rc = fail(c, ADGPU_ERR_HIP, std::string("hipMemcpyAsync(ligand): ") + hipGetErrorString(e));
+    }
+    if (!rc) rc = job_start(j, sp, lig.hd.num_genes, init_pop, seeds);   // waits for the copies
+    if (rc) {
+        const std::string msg = t_err;
+        slot_unbind(j);
+        return fail(c, rc, msg);
+    }
+    return ADGPU_OK;
+}
+
+void slot_unbind(adgpu_job* j) {
+    if (!j->counted) return;
+    std::lock_guard<std::mutex> lk(j->ctx->mu);
+    auto mi = j->ctx->maps.find(j->map_id);
+    if (mi != j->ctx->maps.end()) mi->second.jobs--;
+    j->counted = false;
+}
+
+int slot_results(adgpu_job* j, float* best_geno, float* best_e, double* best_f64, float* pop, float* ene) {
+    adgpu_context* c = j->ctx;
+    const int nrun = j->sp.num_runs, pz = j->sp.pop_size;
+    const size_t ne = (size_t)nrun * pz;
+    std::vector<float> P(ne * kGenes), E(ne);
+    int rc = adgpu_job_download(j, P.data(), E.data(), nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<float> geno((size_t)nrun * kGenes);
+    for (int r = 0; r < nrun; ++r) {
+        int bi = 0;
+        for (int k = 1; k < pz; ++k)
+            if (E[(size_t)r * pz + k] < E[(size_t)r * pz + bi]) bi = k;
+        std::memcpy(&geno[(size_t)r * kGenes], &P[((size_t)r * pz + bi) * kGenes], kGenes * sizeof(float));
+        if (best_e) best_e[r] = E[(size_t)r * pz + bi];
+    }
+    if (best_geno) std::memcpy(best_geno, geno.data(), geno.size() * sizeof(float));
+    if (pop) std::memcpy(pop, P.data(), P.size() * sizeof(float));
+    if (ene) std::memcpy(ene, E.data(), E.size() * sizeof(float));
+    if (!best_f64) return ADGPU_OK;
+    const LigandSet* ls; const MapSet* ms;
+    if ((rc = job_lookup(j, &ls, &ms))) return rc;
+    if (nrun > j->best_cap) {
+        (void)hipFree(j->d_best); (void)hipFree(j->d_f64);
+        j->d_best = nullptr;
+        j->d_f64 = nullptr;
+        j->best_cap = 0;
+        if (hipMalloc(&j->d_best, (size_t)nrun * kGenes * sizeof(float)) != hipSuccess ||
+            hipMalloc(&j->d_f64, (size_t)nrun * 3 * sizeof(double)) != hipSuccess)
+            return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(re-scoring scratch) failed");
+        j->best_cap = nrun;
+    }
+    hipStream_t s = j->groups[0].stream;
+    HIPCHK(c, hipMemcpyAsync(j->d_best, geno.data(), geno.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, launch_rescore_f64(launch_args(c, *ls, *ms, s), j->d_best, nrun, j->d_f64));
+    HIPCHK(c, hipMemcpyAsync(best_f64, j->d_f64, (size_t)nrun * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return ADGPU_OK;
+}
+
+}  // namespace adgpu

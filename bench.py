@@ -12,6 +12,15 @@ evaluation budget), inputs resident in HBM.
 Prints one JSON line: evals/s (value), ligands/hour, p50/p99 per-ligand
 latency, the roofline of the dominant kernel (the local-search launch) and a
 CPU baseline (the oracle's LGA on the host cores, bounded sample).
+
+--workload serve measures the serving loop instead: seeded ligands of uneven
+size (atoms lognormal with median 35 and at most 90, torsions 0-25) docked
+against three resident FP16 map sets (60^3, 80^3, 100^3 at 0.375 A), FIRE
+local search, 20 runs x 1 M evaluations per ligand, pop 150.  Three phases:
+all ligands submitted at once (throughput at saturation), Poisson arrivals at
+a fixed rate below it (p50/p99 latency from submission to result), and the
+plain form -- FP32 maps, one ligand at a time through the job API -- on the
+first ligands of the same list, with the median best energies of both forms.
 """
 from __future__ import annotations
 
@@ -55,6 +64,17 @@ def parse():
     # default is 4): with 8 the library runs the job's runs in four groups on
     # four streams, each on its own queue
     ap.add_argument("--hw-queues", type=int, default=8, choices=range(1, 33), metavar="1..32")
+    ap.add_argument("--workload", default="lga", choices=["lga", "serve"])
+    ap.add_argument("--serve-ligands", type=int, default=300)
+    ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-evals", type=int, default=1000000)
+    ap.add_argument("--serve-nrun", type=int, default=20)
+    ap.add_argument("--serve-qps", type=float, default=0.0,
+                    help="arrival rate of the latency phase (0: 70 %% of the measured saturation rate)")
+    ap.add_argument("--serve-plain", type=int, default=12,
+                    help="ligands docked one at a time with FP32 maps for the plain-form baseline")
+    ap.add_argument("--serve-phases", default="saturation,poisson,plain",
+                    help="comma-separated subset of saturation, poisson, plain")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy kernel (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -133,6 +153,131 @@ def cpu_baseline(maps, tables, ep, sp_gpu, seconds):
                        % (nrun, sp.pop_size, gens, evals, dt))
 
 
+def _serve_ligand(k):
+    """Ligand k of the serving workload: atoms lognormal (median 35, 8..90),
+    torsions uniform in 0..25 (fewer when the seeded tree has fewer rotatable
+    bonds)."""
+    from adgpu import synth
+    rng = np.random.default_rng(90000 + k)
+    n_atoms = int(np.clip(np.round(rng.lognormal(np.log(35.0), 0.35)), 8, 90))
+    n_tors = int(rng.integers(0, 26))
+    for t in range(n_tors, -1, -1):
+        try:
+            return synth.make_ligand(n_atoms, t, seed=90000 + k)
+        except RuntimeError:
+            continue
+    raise RuntimeError("no ligand for seed %d" % k)
+
+
+def serve(a):
+    """The serving-loop measurement (module docstring)."""
+    import multiprocessing as mp
+    from adgpu import synth
+    from adgpu._abi import energy_params
+    t_prep = time.perf_counter()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:   # before any HIP call
+        ligs = pool.map(_serve_ligand, range(a.serve_ligands))
+    types = sorted({t for lig in ligs for t in lig.type_names})
+    map_sets = [synth.make_maps(types, npts=n, spacing=0.375, seed=70 + n) for n in (60, 80, 100)]
+    ep = energy_params()
+    reqs = []
+    for k, lig in enumerate(ligs):
+        ms = map_sets[k % 3]
+        tables = synth.ligand_tables(lig, ms.spacing, ms.type_names)
+        init, seeds = initial_state(5000 + k, a.serve_nrun, a.pop, ms)
+        reqs.append((k % 3, tables, init, seeds))
+    print("prepared %d ligands in %.1f s" % (len(reqs), time.perf_counter() - t_prep), file=sys.stderr, flush=True)
+    import adgpu
+    ctx = adgpu.Context(0)
+    sp = adgpu.search_params(num_runs=a.serve_nrun, pop_size=a.pop, max_evals=a.serve_evals, ls="fire",
+                             spacing=0.375)
+    mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
+
+    def run_served(subset, qps, seed):
+        """Submits `subset` (all at once, or Poisson arrivals at qps) and
+        collects every result; returns (wall s, results, stats)."""
+        rng = np.random.default_rng(seed)
+        with adgpu.Server(ctx, slots=a.serve_slots) as srv:
+            t0 = time.perf_counter()
+            tickets = []
+            next_t = 0.0
+            for m, tables, init, seeds in subset:
+                if qps > 0:
+                    next_t += rng.exponential(1.0 / qps)
+                    delay = t0 + next_t - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                tickets.append(srv.submit(mids16[m], tables, sp, init, seeds, ep=ep))
+            res = [srv.result(t) for t in tickets]
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        bad = [r["message"] for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError("served requests failed: %s" % bad[:3])
+        return wall, res, st
+
+    # warm-up: kernels loaded, slots' buffers allocated
+    run_served(reqs[:a.serve_slots], 0.0, 1)
+    wall, res, st = run_served(reqs, 0.0, 2)
+    evals = float(sum(r["total_evals"] for r in res))
+    sat_lph = 3600.0 * len(res) / wall
+    print("saturation: %d ligands in %.2f s, %.0f ligands/h, %.1f M evals/s, peak active %d" %
+          (len(res), wall, sat_lph, evals / wall / 1e6, st["peak_active"]), file=sys.stderr, flush=True)
+    qps = a.serve_qps if a.serve_qps > 0 else 0.7 * len(res) / wall
+    wall_q, res_q, st_q = run_served(reqs, qps, 3)
+    lat = np.array([(r["t_done_ms"] - r["t_submit_ms"]) / 1000.0 for r in res_q])
+    queue = np.array([(r["t_admit_ms"] - r["t_submit_ms"]) / 1000.0 for r in res_q])
+    print("poisson %.2f/s: p50 %.2f s, p99 %.2f s" % (qps, np.percentile(lat, 50), np.percentile(lat, 99)),
+          file=sys.stderr, flush=True)
+    # the plain form: FP32 maps, one ligand at a time through the job API
+    mids32 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F32) for m in map_sets]
+    plain_t, plain_best = [], []
+    for m, tables, init, seeds in reqs[:a.serve_plain]:
+        lid = ctx.ligand_upload(tables, ep)
+        t0 = time.perf_counter()
+        job = ctx.job(mids32[m], lid, sp, init, seeds)
+        job.initpop()
+        job.run()
+        b = job.best_f64()
+        plain_t.append(time.perf_counter() - t0)
+        plain_best.append(float(np.min(b["energy_f32"])))
+        job.destroy()
+        ctx.ligand_release(lid)
+    served_best = [float(np.min(r["best_energies"])) for r in res[:a.serve_plain]]
+    plain_lph = 3600.0 / float(np.mean(plain_t))
+    print("plain FP32 one at a time: %.2f s per ligand, %.0f ligands/h" % (np.mean(plain_t), plain_lph),
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "ligands/hour of the serving loop (FP16 maps, concurrent slots); p50/p99 ligand latency",
+        "value": sat_lph,
+        "unit": "ligands/h",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 (FP16 map storage)",
+        "data": "synthetic (seeded ligands and maps generated in-process)",
+        "config": {"workload": "serving loop: %d ligands (atoms lognormal median 35, max 90; torsions 0-25) on "
+                               "three resident FP16 map sets (60^3, 80^3, 100^3), FIRE, nrun %d x %d evals, pop %d"
+                               % (len(reqs), a.serve_nrun, a.serve_evals, a.pop),
+                   "slots": a.serve_slots, "hw_queues": a.hw_queues},
+        "saturation": {"wall_s": wall, "ligands_per_hour": sat_lph, "evals_per_s": evals / wall,
+                       "peak_active": st["peak_active"], "slot_reallocations": st["slot_reallocations"]},
+        "poisson": {"qps": qps, "wall_s": wall_q, "ligands_per_hour": 3600.0 * len(res_q) / wall_q,
+                    "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                    "queue_s": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+                    "peak_active": st_q["peak_active"]},
+        "plain_f32_one_at_a_time": {"ligands": len(plain_t), "s_per_ligand": float(np.mean(plain_t)),
+                                    "ligands_per_hour": plain_lph,
+                                    "latency_s": {"p50": float(np.percentile(plain_t, 50)),
+                                                  "p99": float(np.percentile(plain_t, 99))}},
+        "speedup_vs_plain": sat_lph / plain_lph,
+        "median_best_energy": {"served_f16": float(np.median(served_best)),
+                               "plain_f32": float(np.median(plain_best)),
+                               "ligands": len(plain_best)},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -144,6 +289,9 @@ def main():
               "exchange between GPUs during the search, so it runs one process per GPU; "
               "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "serve":
+        serve(a)
+        return
     import adgpu
     lig, maps, tables, ep = build_inputs()
     fmt = {"f32": adgpu.MAP_F32, "f16": adgpu.MAP_F16, "bf16": adgpu.MAP_BF16}[a.maps]

@@ -47,7 +47,8 @@ typedef enum adgpu_status {
     ADGPU_ERR_LIMIT     = 2,   /* a size exceeds one of the limits above        */
     ADGPU_ERR_HIP       = 3,   /* a HIP runtime call failed                     */
     ADGPU_ERR_NOMEM     = 4,   /* device memory could not be allocated          */
-    ADGPU_ERR_STATE     = 5    /* call not valid in the current object state    */
+    ADGPU_ERR_STATE     = 5,   /* call not valid in the current object state    */
+    ADGPU_ERR_TIMEOUT   = 6    /* a wait ended before the request finished      */
 } adgpu_status;
 
 typedef enum adgpu_map_format {
@@ -156,6 +157,8 @@ int  adgpu_device_count(int* out_count);
  * (host/inc/performdocking.h.Cuda:65-75, host/src/performdocking.cpp.Cuda:117-222). */
 int  adgpu_open(int device, adgpu_ctx* out);
 int  adgpu_close(adgpu_ctx ctx);
+/* Message of the calling thread's last failed call (the reference prints it
+ * and exits, cuda/GpuData.h:40-72). */
 const char* adgpu_last_error(adgpu_ctx ctx);
 /* Version string; also usable without a GPU. */
 const char* adgpu_version(void);
@@ -291,6 +294,89 @@ int  adgpu_job_autostop_check(adgpu_job_t job, adgpu_autostop_record* out);
 int  adgpu_job_autostop_log(adgpu_job_t job, int first, int max, adgpu_autostop_record* out, int* out_count);
 /* The end-of-docking statistics (output_final_stddev, autostop.hpp:233-244). */
 int  adgpu_job_autostop_final(adgpu_job_t job, adgpu_autostop_record* out);
+
+/* ---- serving loop -----------------------------------------------------------
+ * Docking requests (a ligand against one of the context's resident map sets)
+ * arrive one by one and run concurrently.  The reference docks the ligands
+ * of a file list one after another per device (main.cpp:135-187,312-327,
+ * 401-434; the OVERLAP pipeline only overlaps a ligand's setup with the
+ * previous docking).  Here a fixed number of slots run requests at the same
+ * time, each on its own streams, so that the generations of different
+ * ligands fill the device together; a request is admitted, in arrival order,
+ * when a slot is free and its device memory fits the budget.  A slot keeps
+ * its buffers for the next request (no allocation in steady state), and a
+ * request's result does not depend on what else runs: every search is
+ * seeded by its own request. */
+typedef struct adgpu_server* adgpu_server_t;
+
+typedef struct adgpu_server_config {
+    int32_t slots;            /* requests on the device at once (1..256)             */
+    int32_t run_groups;       /* streams per request (0 = 1; at most one per run)     */
+    int64_t memory_budget;    /* device bytes for admitted requests (0 = 90 % of the
+                                 free device memory when the server starts)           */
+} adgpu_server_config;
+
+#define ADGPU_REQ_KEEP_POPULATION  1   /* keep the final populations and energies   */
+#define ADGPU_REQ_RESCORE_F64      2   /* FP64 re-scoring of each run's best pose   */
+
+/* One docking: everything adgpu_ligand_upload + adgpu_job_create take, plus
+ * AutoStop (adgpu_job_set_autostop) and the result options.  The server
+ * copies every array before adgpu_server_submit returns. */
+typedef struct adgpu_request {
+    int32_t                     map_id;
+    const adgpu_ligand*         ligand;
+    const adgpu_energy_params*  energy;
+    const adgpu_search_params*  search;
+    const float*                init_population;   /* num_runs*pop_size*64            */
+    const uint32_t*             prng_seeds;        /* num_runs*pop_size*64            */
+    int32_t                     autostop;          /* 1 = AutoStop on                 */
+    int32_t                     as_frequency;
+    float                       stopstd;
+    int64_t                     min_as_evals;
+    int32_t                     flags;             /* ADGPU_REQ_* bits                */
+} adgpu_request;
+
+typedef struct adgpu_request_result {
+    int32_t status;           /* adgpu_status of the docking                         */
+    int32_t slot;             /* slot that ran it                                    */
+    int32_t num_runs, pop_size;
+    int64_t generations;      /* generation_cnt                                      */
+    int64_t total_evals;      /* sum of evals_of_runs                                */
+    int64_t device_bytes;     /* memory charged against the budget                   */
+    double  t_submit_ms;      /* server clock: milliseconds since adgpu_server_create */
+    double  t_admit_ms;
+    double  t_done_ms;
+    char    message[160];     /* error text when status != 0                         */
+} adgpu_request_result;
+
+typedef struct adgpu_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t queued, active, peak_active;
+    int64_t bytes_in_use, peak_bytes, memory_budget;
+    int64_t slot_reallocations;   /* slot buffer growths after their first allocation */
+    double  now_ms;
+} adgpu_server_stats;
+
+int  adgpu_server_create(adgpu_ctx ctx, const adgpu_server_config* cfg, adgpu_server_t* out);
+/* Queues a request and returns its ticket.  Fails at once (nothing queued)
+ * for an invalid ligand or search, an unknown map set, or a request larger
+ * than the whole memory budget (ADGPU_ERR_LIMIT). */
+int  adgpu_server_submit(adgpu_server_t srv, const adgpu_request* req, int64_t* ticket);
+/* Oldest finished request not yet returned here; waits up to timeout_ms
+ * (< 0: no limit) and returns ADGPU_ERR_TIMEOUT when none finished. */
+int  adgpu_server_next_done(adgpu_server_t srv, int timeout_ms, int64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for the request, copies its result
+ * and forgets the ticket.  best_genotypes: num_runs*64 floats, best_energies:
+ * num_runs, best_f64: 3*num_runs doubles (total, inter, intra; needs
+ * ADGPU_REQ_RESCORE_F64), population: num_runs*pop_size*64 and energies:
+ * num_runs*pop_size (need ADGPU_REQ_KEEP_POPULATION).  Any pointer may be
+ * null. */
+int  adgpu_server_result(adgpu_server_t srv, int64_t ticket, int timeout_ms, adgpu_request_result* out,
+                         float* best_genotypes, float* best_energies, double* best_f64,
+                         float* population, float* energies);
+int  adgpu_server_stats_get(adgpu_server_t srv, adgpu_server_stats* out);
+/* Finishes the running requests, drops the queued ones and frees the slots. */
+int  adgpu_server_destroy(adgpu_server_t srv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,168 @@
+"""Serving loop (adgpu_server_*) through the C ABI.
+
+* Each request's result is the same alone and in the mix.  Uneven ligands
+  (10-90 atoms, 0-25 torsions) against three resident map sets (FP32 and
+  FP16), all four local searches, one request with AutoStop, are docked
+  one at a time through the plain job API, then all at once through a
+  server whose slots run them concurrently.  Per-run best genotypes and
+  energies, generation counts and eval totals must be bit-identical: every
+  search is seeded by its own request, and nothing of one request reaches
+  another's kernels.
+* Admission by memory: a budget that holds only a few requests keeps the
+  bytes in use under it at every moment, and the results are again the same.
+* The per-run best energies equal the oracle's FP32 energy of the returned
+  genotypes (the bar of helpers.check_energies), and the FP64 re-scoring of
+  each run's best pose equals the FP64 oracle to 1e-10 of the sum of |terms|.
+* Refused at submit: a request larger than the whole budget, an unknown map
+  set; a map set in use by a queued or running request cannot be released
+  while that request runs.
+"""
+import numpy as np
+import pytest
+
+import adgpu
+import helpers
+import oracle as O
+from adgpu import synth
+from adgpu._abi import CGrid, CLigand, energy_params, search_params
+
+pytestmark = pytest.mark.gpu
+
+NRUN, POP = 4, 40
+
+
+def _ligand(n_atoms, n_tors, seed):
+    for t in range(n_tors, -1, -1):
+        try:
+            return synth.make_ligand(n_atoms, t, seed=seed)
+        except RuntimeError:
+            continue
+    raise RuntimeError("no ligand")
+
+
+def _rounded(packed, fmt):
+    if fmt == adgpu.MAP_F16:
+        return np.clip(packed, -65504.0, 65504.0).astype(np.float16).astype(np.float32)
+    return packed
+
+
+@pytest.fixture(scope="module")
+def mix(gpu_ctx):
+    """Ten requests over three resident map sets."""
+    rng = np.random.default_rng(4242)
+    atoms = [10, 90] + [int(a) for a in rng.integers(11, 90, 8)]
+    ligs = [_ligand(a, int(rng.integers(0, 26)), 500 + k) for k, a in enumerate(atoms)]
+    types = sorted({t for lig in ligs for t in lig.type_names})
+    sets = []
+    for npts, seed, fmt in ((40, 51, adgpu.MAP_F32), (48, 52, adgpu.MAP_F16), (56, 53, adgpu.MAP_F32)):
+        maps = synth.make_maps(types, npts=npts, seed=seed)
+        packed = maps.packed()
+        mid = gpu_ctx.maps_upload(packed, maps.size, maps.num_map_atypes, maps.spacing, fmt)
+        sets.append((maps, mid, CGrid(_rounded(packed, fmt), maps.size, maps.num_map_atypes, maps.spacing)))
+    ep = energy_params()
+    methods = ["ad", "sw", "adam", "fire"]
+    reqs = []
+    for k, lig in enumerate(ligs):
+        maps, mid, cg = sets[k % 3]
+        T = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+        sp = search_params(num_runs=NRUN, pop_size=POP, max_evals=12000, ls=methods[k % 4], spacing=maps.spacing)
+        init = O.initial_population((k, 17), NRUN, POP, maps.size_angstr, maps.spacing)
+        seeds = O.cmwc_uints((k, 17), NRUN * POP * 64)
+        autostop = (0.15, 5, 0) if k == 3 else None
+        reqs.append(dict(k=k, mid=mid, T=T, sp=sp, init=init, seeds=seeds, autostop=autostop, cg=cg, ep=ep))
+    yield reqs
+    for _, mid, _ in sets:
+        gpu_ctx.maps_release(mid)
+
+
+def _alone(ctx, q):
+    """One request through the plain job API (its own ligand upload)."""
+    lid = ctx.ligand_upload(q["T"], q["ep"])
+    job = ctx.job(q["mid"], lid, q["sp"], q["init"], q["seeds"])
+    if q["autostop"] is not None:
+        job.set_autostop(True, *q["autostop"])
+    job.initpop()
+    gens, total = job.run()
+    b = job.best_f64()
+    job.destroy()
+    ctx.ligand_release(lid)
+    return dict(generations=gens, total_evals=total, best_genotypes=b["genotypes"], best_energies=b["energy_f32"],
+                best_f64=np.stack([b["total"], b["inter"], b["intra"]], axis=1))
+
+
+def _serve(ctx, reqs, slots, budget=0):
+    with adgpu.Server(ctx, slots=slots, memory_budget=budget) as srv:
+        tickets = [srv.submit(q["mid"], q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"], autostop=q["autostop"],
+                              rescore_f64=True) for q in reqs]
+        print("submitted", len(tickets), "slots", slots, "budget", budget, flush=True)
+        out = [srv.result(t) for t in tickets]
+        stats = srv.stats()
+    return out, stats
+
+
+def _same(a, b, k):
+    assert a["generations"] == b["generations"], (k, a["generations"], b["generations"])
+    assert a["total_evals"] == b["total_evals"], (k, a["total_evals"], b["total_evals"])
+    assert np.array_equal(a["best_energies"], b["best_energies"]), k
+    assert np.array_equal(a["best_genotypes"], b["best_genotypes"]), k
+    assert np.array_equal(a["best_f64"], b["best_f64"]), k
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix(gpu_ctx, mix):
+    alone = []
+    for q in mix:
+        print("alone", q["k"], "atoms", q["T"].num_atoms, "genes", q["T"].num_rotbonds + 6, flush=True)
+        alone.append(_alone(gpu_ctx, q))
+    served, stats = _serve(gpu_ctx, mix, slots=5)
+    print("stats", stats, flush=True)
+    assert stats["completed"] == len(mix) and stats["failed"] == 0
+    assert stats["peak_active"] >= 2          # the requests overlapped on the device
+    for q, a, s in zip(mix, alone, served):
+        assert s["status"] == 0, s["message"]
+        assert s["t_submit_ms"] <= s["t_admit_ms"] <= s["t_done_ms"]
+        _same(a, s, q["k"])
+    # the AutoStop request ran under AutoStop: at most the generations of the plain search
+    assert served[3]["total_evals"] >= NRUN * 12000 or served[3]["generations"] % 5 == 0
+
+
+def test_admission_by_memory_keeps_bytes_under_the_budget(gpu_ctx, mix):
+    ref, _ = _serve(gpu_ctx, mix, slots=1)
+    sizes = sorted(r["device_bytes"] for r in ref)
+    budget = sizes[-1] + sizes[-2]            # any two requests fit, not every three
+    served, stats = _serve(gpu_ctx, mix, slots=8, budget=budget)
+    print("stats", stats, "sizes", sizes, flush=True)
+    assert stats["peak_bytes"] <= budget
+    fit = 0
+    while fit < len(sizes) and sum(sizes[:fit + 1]) <= budget:
+        fit += 1
+    assert 2 <= stats["peak_active"] <= fit
+    for q, a, s in zip(mix, ref, served):
+        assert s["status"] == 0, s["message"]
+        _same(a, s, q["k"])
+
+
+def test_served_results_match_the_oracle(gpu_ctx, mix):
+    served, _ = _serve(gpu_ctx, mix, slots=4)
+    for q, s in zip(mix, served):
+        orc = O.Oracle(q["cg"], CLigand(q["T"]), q["ep"])
+        helpers.check_energies(orc, s["best_genotypes"], s["best_energies"], edge_share=1.0)
+        for r in range(NRUN):
+            e64, ei, ea, sc = orc.energy64(s["best_genotypes"][r])
+            tol = 1e-10 * max(1.0, sc)
+            assert abs(s["best_f64"][r, 0] - e64) <= tol, (q["k"], r, s["best_f64"][r], e64)
+            assert abs(s["best_f64"][r, 1] - ei) <= tol and abs(s["best_f64"][r, 2] - ea) <= tol
+
+
+def test_refusals(gpu_ctx, mix):
+    q = mix[0]
+    with adgpu.Server(gpu_ctx, slots=2, memory_budget=4096) as srv:
+        with pytest.raises(adgpu.AdgpuError, match="budget"):
+            srv.submit(q["mid"], q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"])
+    with adgpu.Server(gpu_ctx, slots=2) as srv:
+        with pytest.raises(adgpu.AdgpuError, match="map id"):
+            srv.submit(987654, q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"])
+        t = srv.submit(q["mid"], q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"])
+        r = srv.result(t)
+        assert r["status"] == 0
+        assert srv.next_done(0) is None       # result() already took it
+        assert srv.stats()["completed"] == 1
