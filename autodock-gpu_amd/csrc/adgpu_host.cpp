@@ -76,6 +76,11 @@ struct RunGroup {
 };
 
 constexpr int kMaxGroups = 8;
+constexpr int64_t kGroupSearches = 1500;   // local searches per generation per default run group
+
+// Flags of the events the host waits on: blocking (the thread sleeps until
+// the device signals) when ADGPU_HOST_WAIT=block, else the runtime's wait.
+static std::atomic<unsigned> g_wait_event_flags{hipEventDisableTiming};
 
 struct adgpu_job {
     adgpu_context* ctx = nullptr;
@@ -354,6 +359,19 @@ int adgpu_open(int device, adgpu_ctx* out) {
     std::unique_ptr<adgpu_context> c(new adgpu_context());
     c->device = device;
     if (hipSetDevice(device) != hipSuccess) return ADGPU_ERR_HIP;
+    // How host threads wait for the device (ADGPU_HOST_WAIT = spin, yield or
+    // block): many threads waiting at once (the serving loop's workers) share
+    // the host cores.  Applies to the process; the runtime's default otherwise.
+    if (const char* w = std::getenv("ADGPU_HOST_WAIT")) {
+        const std::string v(w);
+        const unsigned f = v == "spin" ? hipDeviceScheduleSpin : v == "yield" ? hipDeviceScheduleYield
+                         : v == "block" ? hipDeviceScheduleBlockingSync : hipDeviceScheduleAuto;
+        if (hipSetDeviceFlags(f) != hipSuccess) {
+            (void)hipGetLastError();
+            std::fprintf(stderr, "adgpu: ADGPU_HOST_WAIT=%s not applied (device already initialized)\n", w);
+        }
+        g_wait_event_flags = hipEventDisableTiming | (v == "block" ? hipEventBlockingSync : 0u);
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
     if (const char* tw = std::getenv("ADGPU_TEAM_WIDTH")) {
         int v = std::atoi(tw);
@@ -545,7 +563,7 @@ int adgpu_rescore_f64(adgpu_ctx c, int map_id, int lig_id, const float* genotype
 static bool make_group(RunGroup& g) {
     bool ok = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) == hipSuccess;
     for (int r = 0; r < kRing && ok; ++r)
-        ok = hipEventCreateWithFlags(&g.ev_done[r], hipEventDisableTiming) == hipSuccess &&
+        ok = hipEventCreateWithFlags(&g.ev_done[r], g_wait_event_flags.load()) == hipSuccess &&
              hipEventCreate(&g.ev_gen[0][r]) == hipSuccess && hipEventCreate(&g.ev_gen[1][r]) == hipSuccess &&
              hipEventCreate(&g.ev_ls[0][r]) == hipSuccess && hipEventCreate(&g.ev_ls[1][r]) == hipSuccess;
     return ok;
@@ -587,7 +605,7 @@ static int job_open(adgpu_job* j) {
     bool ok = hipMalloc(&j->counters, 16 * kMaxGroups * sizeof(int32_t)) == hipSuccess &&
               hipMalloc(&j->d_stop, 16 * sizeof(int32_t)) == hipSuccess &&
               hipHostMalloc(&j->h_stop, 16 * sizeof(int32_t)) == hipSuccess &&
-              hipEventCreateWithFlags(&j->ev_test, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&j->ev_test, g_wait_event_flags.load()) == hipSuccess &&
               hipEventCreate(&j->ev_origin) == hipSuccess;
     return ok ? ADGPU_OK : fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
 }
@@ -704,10 +722,16 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     // hardware queue serialize, so the default follows the HIP runtime's
     // hardware queues per process (GPU_MAX_HW_QUEUES, 4 unless set): 2 groups
     // (with the context's stream, 3 of 4 queues), or 4 groups when at least
-    // 6 queues are available (the faster setting on MI355X)
+    // 6 queues are available (the faster setting on MI355X).  A group's
+    // local-search launch should still fill a good part of the device, so
+    // small jobs get fewer groups: one per kGroupSearches searches of a
+    // generation (measured: nrun 20 x 150 searches fastest in 2 groups,
+    // nrun 50 x 150 in 4)
     int ng = 2;
     if (const char* q = std::getenv("GPU_MAX_HW_QUEUES"))
         if (std::atoi(q) >= 6) ng = 4;
+    const int64_t searches = (int64_t)nrun * std::max(1, to_dev(sp, num_genes).num_ls);
+    ng = (int)std::max<int64_t>(1, std::min<int64_t>(ng, searches / kGroupSearches));
     if (const char* e = std::getenv("ADGPU_RUN_GROUPS")) ng = std::atoi(e);
     rc = job_open(j.get());
     if (!rc) rc = job_set_groups(j.get(), ng, nrun);

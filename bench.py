@@ -13,7 +13,9 @@ Prints one JSON line: evals/s (value), ligands/hour, p50/p99 per-ligand
 latency, the roofline of the dominant kernel (the local-search launch) and a
 CPU baseline (the oracle's LGA on the host cores, bounded sample).
 
---workload serve measures the serving loop instead: seeded ligands of uneven
+--workload screen measures a batched screen (seeded ligands of 10-60 atoms
+and 0-20 torsions, one 80^3 map set, ADAM, nrun 50) through the same server
+with FP32 and FP16 maps.  --workload serve measures the serving loop instead: seeded ligands of uneven
 size (atoms lognormal with median 35 and at most 90, torsions 0-25) docked
 against three resident FP16 map sets (60^3, 80^3, 100^3 at 0.375 A), FIRE
 local search, 20 runs x 1 M evaluations per ligand, pop 150.  Three phases:
@@ -64,7 +66,8 @@ def parse():
     # default is 4): with 8 the library runs the job's runs in four groups on
     # four streams, each on its own queue
     ap.add_argument("--hw-queues", type=int, default=8, choices=range(1, 33), metavar="1..32")
-    ap.add_argument("--workload", default="lga", choices=["lga", "serve"])
+    ap.add_argument("--workload", default="lga", choices=["lga", "serve", "screen"])
+    ap.add_argument("--screen-ligands", type=int, default=64)
     ap.add_argument("--serve-ligands", type=int, default=300)
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-evals", type=int, default=1000000)
@@ -73,8 +76,9 @@ def parse():
                     help="arrival rate of the latency phase (0: 70 %% of the measured saturation rate)")
     ap.add_argument("--serve-plain", type=int, default=12,
                     help="ligands docked one at a time with FP32 maps for the plain-form baseline")
-    ap.add_argument("--serve-phases", default="saturation,poisson,plain",
-                    help="comma-separated subset of saturation, poisson, plain")
+    ap.add_argument("--serve-phases", default="saturation,poisson,f32,plain",
+                    help="comma-separated subset of saturation, poisson, f32 (the same requests on FP32 "
+                         "maps), plain")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy kernel (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -153,103 +157,154 @@ def cpu_baseline(maps, tables, ep, sp_gpu, seconds):
                        % (nrun, sp.pop_size, gens, evals, dt))
 
 
-def _serve_ligand(k):
+def _lognormal_ligand(k):
     """Ligand k of the serving workload: atoms lognormal (median 35, 8..90),
     torsions uniform in 0..25 (fewer when the seeded tree has fewer rotatable
     bonds)."""
-    from adgpu import synth
     rng = np.random.default_rng(90000 + k)
     n_atoms = int(np.clip(np.round(rng.lognormal(np.log(35.0), 0.35)), 8, 90))
-    n_tors = int(rng.integers(0, 26))
+    return _ligand_with(n_atoms, int(rng.integers(0, 26)), 90000 + k)
+
+
+def _uniform_ligand(k):
+    """Ligand k of the screening workload: atoms uniform in 10..60, torsions
+    uniform in 0..20."""
+    rng = np.random.default_rng(60000 + k)
+    return _ligand_with(int(rng.integers(10, 61)), int(rng.integers(0, 21)), 60000 + k)
+
+
+def _ligand_with(n_atoms, n_tors, seed):
+    from adgpu import synth
     for t in range(n_tors, -1, -1):
         try:
-            return synth.make_ligand(n_atoms, t, seed=90000 + k)
+            return synth.make_ligand(n_atoms, t, seed=seed)
         except RuntimeError:
             continue
-    raise RuntimeError("no ligand for seed %d" % k)
+    raise RuntimeError("no ligand for seed %d" % seed)
+
+
+def _prepare(make, n, map_points, nrun, pop):
+    """n seeded ligands (made in a process pool, before any HIP call), map
+    sets of the given sizes over their atom types, and each ligand's tables,
+    initial population and LCG seeds; ligand k goes to map set k mod len."""
+    import multiprocessing as mp
+    from adgpu import synth
+    t_prep = time.perf_counter()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        ligs = pool.map(make, range(n))
+    types = sorted({t for lig in ligs for t in lig.type_names})
+    map_sets = [synth.make_maps(types, npts=p, spacing=0.375, seed=70 + p) for p in map_points]
+    reqs = []
+    for k, lig in enumerate(ligs):
+        ms = map_sets[k % len(map_sets)]
+        tables = synth.ligand_tables(lig, ms.spacing, ms.type_names)
+        init, seeds = initial_state(5000 + k, nrun, pop, ms)
+        reqs.append((k % len(map_sets), tables, init, seeds))
+    print("prepared %d ligands in %.1f s" % (len(reqs), time.perf_counter() - t_prep), file=sys.stderr, flush=True)
+    return map_sets, reqs
+
+
+def _run_served(ctx, slots, reqs, mids, sp, ep, qps=0.0, seed=0):
+    """Submits the requests to a server (all at once, or Poisson arrivals at
+    qps per second) and collects every result; returns (wall s, results,
+    server statistics with the host CPU seconds used)."""
+    import adgpu
+    rng = np.random.default_rng(seed)
+    with adgpu.Server(ctx, slots=slots) as srv:
+        cpu0 = os.times()
+        t0 = time.perf_counter()
+        tickets = []
+        next_t = 0.0
+        for m, tables, init, seeds in reqs:
+            if qps > 0:
+                next_t += rng.exponential(1.0 / qps)
+                delay = t0 + next_t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            tickets.append(srv.submit(mids[m], tables, sp, init, seeds, ep=ep))
+        res = [srv.result(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        cpu1 = os.times()
+        st = srv.stats()
+        st["host_cpu_s"] = (cpu1.user - cpu0.user) + (cpu1.system - cpu0.system)
+    bad = [r["message"] for r in res if r["status"] != 0]
+    if bad:
+        raise RuntimeError("served requests failed: %s" % bad[:3])
+    return wall, res, st
+
+
+def _saturation(ctx, slots, reqs, mids, sp, ep, label):
+    wall, res, st = _run_served(ctx, slots, reqs, mids, sp, ep)
+    evals = float(sum(r["total_evals"] for r in res))
+    lph = 3600.0 * len(res) / wall
+    print("%s: %d ligands in %.2f s, %.0f ligands/h, %.1f M evals/s, peak active %d, host CPU %.1f s" %
+          (label, len(res), wall, lph, evals / wall / 1e6, st["peak_active"], st["host_cpu_s"]),
+          file=sys.stderr, flush=True)
+    return res, {"wall_s": wall, "ligands_per_hour": lph, "evals_per_s": evals / wall,
+                 "host_cpu_s": st["host_cpu_s"], "peak_active": st["peak_active"],
+                 "slot_reallocations": st["slot_reallocations"]}
+
+
+def _plain(ctx, reqs, mids32, sp, ep, out, served_value):
+    """The plain form: FP32 maps, one ligand at a time through the job API,
+    with the default run groups and with one stream (the reference's loop
+    structure); per-ligand FP64 best energies."""
+    for label, groups in (("plain_f32_one_at_a_time", None), ("plain_f32_one_stream", "1")):
+        if groups is None:
+            os.environ.pop("ADGPU_RUN_GROUPS", None)
+        else:
+            os.environ["ADGPU_RUN_GROUPS"] = groups
+        plain_t, plain_best = [], []
+        for m, tables, init, seeds in reqs:
+            lid = ctx.ligand_upload(tables, ep)
+            t0 = time.perf_counter()
+            job = ctx.job(mids32[m], lid, sp, init, seeds)
+            job.initpop()
+            job.run()
+            b = job.best_f64()
+            plain_t.append(time.perf_counter() - t0)
+            plain_best.append(float(np.min(b["total"])))
+            job.destroy()
+            ctx.ligand_release(lid)
+        lph = 3600.0 / float(np.mean(plain_t))
+        print("%s: %.3f s per ligand, %.0f ligands/h" % (label, np.mean(plain_t), lph), file=sys.stderr, flush=True)
+        out[label] = {"ligands": len(plain_t), "s_per_ligand": float(np.mean(plain_t)), "ligands_per_hour": lph,
+                      "latency_s": {"p50": float(np.percentile(plain_t, 50)),
+                                    "p99": float(np.percentile(plain_t, 99))},
+                      "median_best_energy_f64": float(np.median(plain_best))}
+        if served_value:
+            out["speedup_vs_" + label.replace("plain_f32_", "")] = served_value / lph
+    os.environ.pop("ADGPU_RUN_GROUPS", None)
+
+
+def _best_on_f32(ctx, reqs, res, mids32, ep):
+    """Per ligand the lowest FP64 energy of its runs' best poses re-scored on
+    the FP32 maps."""
+    best = []
+    for (m, tables, _, _), r in zip(reqs, res):
+        lid = ctx.ligand_upload(tables, ep)
+        tot, _, _ = ctx.rescore_f64(mids32[m], lid, r["best_genotypes"])
+        best.append(float(np.min(tot)))
+        ctx.ligand_release(lid)
+    return best
 
 
 def serve(a):
     """The serving-loop measurement (module docstring)."""
-    import multiprocessing as mp
-    from adgpu import synth
     from adgpu._abi import energy_params
-    t_prep = time.perf_counter()
-    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:   # before any HIP call
-        ligs = pool.map(_serve_ligand, range(a.serve_ligands))
-    types = sorted({t for lig in ligs for t in lig.type_names})
-    map_sets = [synth.make_maps(types, npts=n, spacing=0.375, seed=70 + n) for n in (60, 80, 100)]
+    map_sets, reqs = _prepare(_lognormal_ligand, a.serve_ligands, (60, 80, 100), a.serve_nrun, a.pop)
     ep = energy_params()
-    reqs = []
-    for k, lig in enumerate(ligs):
-        ms = map_sets[k % 3]
-        tables = synth.ligand_tables(lig, ms.spacing, ms.type_names)
-        init, seeds = initial_state(5000 + k, a.serve_nrun, a.pop, ms)
-        reqs.append((k % 3, tables, init, seeds))
-    print("prepared %d ligands in %.1f s" % (len(reqs), time.perf_counter() - t_prep), file=sys.stderr, flush=True)
+    # the workers wait for the device in the kernel, not by spinning on host cores
+    os.environ.setdefault("ADGPU_HOST_WAIT", "block")
     import adgpu
     ctx = adgpu.Context(0)
     sp = adgpu.search_params(num_runs=a.serve_nrun, pop_size=a.pop, max_evals=a.serve_evals, ls="fire",
                              spacing=0.375)
     mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
-
-    def run_served(subset, qps, seed):
-        """Submits `subset` (all at once, or Poisson arrivals at qps) and
-        collects every result; returns (wall s, results, stats)."""
-        rng = np.random.default_rng(seed)
-        with adgpu.Server(ctx, slots=a.serve_slots) as srv:
-            t0 = time.perf_counter()
-            tickets = []
-            next_t = 0.0
-            for m, tables, init, seeds in subset:
-                if qps > 0:
-                    next_t += rng.exponential(1.0 / qps)
-                    delay = t0 + next_t - time.perf_counter()
-                    if delay > 0:
-                        time.sleep(delay)
-                tickets.append(srv.submit(mids16[m], tables, sp, init, seeds, ep=ep))
-            res = [srv.result(t) for t in tickets]
-            wall = time.perf_counter() - t0
-            st = srv.stats()
-        bad = [r["message"] for r in res if r["status"] != 0]
-        if bad:
-            raise RuntimeError("served requests failed: %s" % bad[:3])
-        return wall, res, st
-
-    # warm-up: kernels loaded, slots' buffers allocated
-    run_served(reqs[:a.serve_slots], 0.0, 1)
-    wall, res, st = run_served(reqs, 0.0, 2)
-    evals = float(sum(r["total_evals"] for r in res))
-    sat_lph = 3600.0 * len(res) / wall
-    print("saturation: %d ligands in %.2f s, %.0f ligands/h, %.1f M evals/s, peak active %d" %
-          (len(res), wall, sat_lph, evals / wall / 1e6, st["peak_active"]), file=sys.stderr, flush=True)
-    qps = a.serve_qps if a.serve_qps > 0 else 0.7 * len(res) / wall
-    wall_q, res_q, st_q = run_served(reqs, qps, 3)
-    lat = np.array([(r["t_done_ms"] - r["t_submit_ms"]) / 1000.0 for r in res_q])
-    queue = np.array([(r["t_admit_ms"] - r["t_submit_ms"]) / 1000.0 for r in res_q])
-    print("poisson %.2f/s: p50 %.2f s, p99 %.2f s" % (qps, np.percentile(lat, 50), np.percentile(lat, 99)),
-          file=sys.stderr, flush=True)
-    # the plain form: FP32 maps, one ligand at a time through the job API
-    mids32 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F32) for m in map_sets]
-    plain_t, plain_best = [], []
-    for m, tables, init, seeds in reqs[:a.serve_plain]:
-        lid = ctx.ligand_upload(tables, ep)
-        t0 = time.perf_counter()
-        job = ctx.job(mids32[m], lid, sp, init, seeds)
-        job.initpop()
-        job.run()
-        b = job.best_f64()
-        plain_t.append(time.perf_counter() - t0)
-        plain_best.append(float(np.min(b["energy_f32"])))
-        job.destroy()
-        ctx.ligand_release(lid)
-    served_best = [float(np.min(r["best_energies"])) for r in res[:a.serve_plain]]
-    plain_lph = 3600.0 / float(np.mean(plain_t))
-    print("plain FP32 one at a time: %.2f s per ligand, %.0f ligands/h" % (np.mean(plain_t), plain_lph),
-          file=sys.stderr, flush=True)
+    phases = set(a.serve_phases.split(","))
     out = {
         "metric": "ligands/hour of the serving loop (FP16 maps, concurrent slots); p50/p99 ligand latency",
-        "value": sat_lph,
+        "value": None,
         "unit": "ligands/h",
         "n_gpus": 1,
         "higher_is_better": True,
@@ -258,22 +313,94 @@ def serve(a):
         "config": {"workload": "serving loop: %d ligands (atoms lognormal median 35, max 90; torsions 0-25) on "
                                "three resident FP16 map sets (60^3, 80^3, 100^3), FIRE, nrun %d x %d evals, pop %d"
                                % (len(reqs), a.serve_nrun, a.serve_evals, a.pop),
-                   "slots": a.serve_slots, "hw_queues": a.hw_queues},
-        "saturation": {"wall_s": wall, "ligands_per_hour": sat_lph, "evals_per_s": evals / wall,
-                       "peak_active": st["peak_active"], "slot_reallocations": st["slot_reallocations"]},
-        "poisson": {"qps": qps, "wall_s": wall_q, "ligands_per_hour": 3600.0 * len(res_q) / wall_q,
-                    "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-                    "queue_s": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
-                    "peak_active": st_q["peak_active"]},
-        "plain_f32_one_at_a_time": {"ligands": len(plain_t), "s_per_ligand": float(np.mean(plain_t)),
-                                    "ligands_per_hour": plain_lph,
-                                    "latency_s": {"p50": float(np.percentile(plain_t, 50)),
-                                                  "p99": float(np.percentile(plain_t, 99))}},
-        "speedup_vs_plain": sat_lph / plain_lph,
-        "median_best_energy": {"served_f16": float(np.median(served_best)),
-                               "plain_f32": float(np.median(plain_best)),
-                               "ligands": len(plain_best)},
+                   "slots": a.serve_slots, "hw_queues": a.hw_queues,
+                   "host_wait": os.environ.get("ADGPU_HOST_WAIT")},
     }
+    _run_served(ctx, a.serve_slots, reqs[:a.serve_slots], mids16, sp, ep)   # warm-up
+    res = []
+    if "saturation" in phases or "poisson" in phases:
+        res, out["saturation"] = _saturation(ctx, a.serve_slots, reqs, mids16, sp, ep, "saturation")
+        out["value"] = out["saturation"]["ligands_per_hour"]
+    if "poisson" in phases:
+        qps = a.serve_qps if a.serve_qps > 0 else 0.7 * out["saturation"]["ligands_per_hour"] / 3600.0
+        wall_q, res_q, st_q = _run_served(ctx, a.serve_slots, reqs, mids16, sp, ep, qps, 3)
+        lat = np.array([(r["t_done_ms"] - r["t_submit_ms"]) / 1000.0 for r in res_q])
+        queue = np.array([(r["t_admit_ms"] - r["t_submit_ms"]) / 1000.0 for r in res_q])
+        print("poisson %.2f/s: p50 %.2f s, p99 %.2f s" % (qps, np.percentile(lat, 50), np.percentile(lat, 99)),
+              file=sys.stderr, flush=True)
+        out["poisson"] = {"qps": qps, "wall_s": wall_q, "ligands_per_hour": 3600.0 * len(res_q) / wall_q,
+                          "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                          "queue_s": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+                          "peak_active": st_q["peak_active"]}
+    mids32 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F32) for m in map_sets]
+    if "f32" in phases and res:
+        # the same requests on FP32 maps: what FP16 storage changes in the results
+        res32, out["saturation_f32"] = _saturation(ctx, a.serve_slots, reqs, mids32, sp, ep, "saturation FP32")
+        b16 = np.array(_best_on_f32(ctx, reqs, res, mids32, ep))
+        b32 = np.array(_best_on_f32(ctx, reqs, res32, mids32, ep))
+        d = b16 - b32
+        out["best_energy_f64"] = {"median_f16": float(np.median(b16)), "median_f32": float(np.median(b32)),
+                                  "paired_diff_mean": float(np.mean(d)),
+                                  "paired_diff_sem": float(np.std(d, ddof=1) / np.sqrt(len(d))),
+                                  "ligands": len(d),
+                                  "note": "per ligand the lowest FP64 energy over its runs' best poses, "
+                                          "re-scored on the FP32 maps"}
+    if "plain" in phases and a.serve_plain > 0:
+        sub = reqs[:a.serve_plain]
+        _plain(ctx, sub, mids32, sp, ep, out, out["value"])
+        if res:
+            served_best = _best_on_f32(ctx, sub, res, mids32, ep)
+            out["median_best_energy_f64"] = {
+                "served_f16": float(np.median(served_best)),
+                "plain_f32": out["plain_f32_one_at_a_time"]["median_best_energy_f64"],
+                "ligands": len(served_best),
+                "note": "per ligand the lowest FP64 energy over its runs' best poses, re-scored on the FP32 maps"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def screen(a):
+    """Batched screen against one receptor: seeded ligands (atoms uniform in
+    10..60, torsions 0..20) on one 80^3 map set, ADAM, nrun 50, pop 150,
+    `--evals` per run, all submitted at once to the server, with FP32 and
+    with FP16 maps; the plain form (FP32, one ligand at a time) on the first
+    ligands, and the paired FP16 - FP32 difference of each ligand's best
+    energy (FP64, re-scored on the FP32 maps)."""
+    from adgpu._abi import energy_params
+    map_sets, reqs = _prepare(_uniform_ligand, a.screen_ligands, (80,), 50, a.pop)
+    ep = energy_params()
+    os.environ.setdefault("ADGPU_HOST_WAIT", "block")
+    import adgpu
+    ctx = adgpu.Context(0)
+    sp = adgpu.search_params(num_runs=50, pop_size=a.pop, max_evals=a.evals, ls="adam", spacing=0.375)
+    mids32 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F32) for m in map_sets]
+    mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
+    out = {
+        "metric": "ligands/hour of a batched screen against one receptor",
+        "value": None,
+        "unit": "ligands/h",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 (FP32 and FP16 map storage)",
+        "data": "synthetic (seeded ligands and maps generated in-process)",
+        "config": {"workload": "batched screen: %d ligands (atoms 10-60, torsions 0-20) on one 80^3 map set, "
+                               "ADAM, nrun 50 x %d evals, pop %d" % (len(reqs), a.evals, a.pop),
+                   "slots": a.serve_slots, "hw_queues": a.hw_queues,
+                   "host_wait": os.environ.get("ADGPU_HOST_WAIT")},
+    }
+    _run_served(ctx, a.serve_slots, reqs[:a.serve_slots], mids16, sp, ep)   # warm-up
+    res32, out["served_f32"] = _saturation(ctx, a.serve_slots, reqs, mids32, sp, ep, "served FP32")
+    res16, out["served_f16"] = _saturation(ctx, a.serve_slots, reqs, mids16, sp, ep, "served FP16")
+    out["value"] = out["served_f16"]["ligands_per_hour"]
+    b32 = _best_on_f32(ctx, reqs, res32, mids32, ep)
+    b16 = _best_on_f32(ctx, reqs, res16, mids32, ep)
+    d = np.array(b16) - np.array(b32)
+    out["best_energy_f64"] = {"median_f32": float(np.median(b32)), "median_f16": float(np.median(b16)),
+                              "paired_diff_mean": float(np.mean(d)),
+                              "paired_diff_sem": float(np.std(d, ddof=1) / np.sqrt(len(d))) if len(d) > 1 else None,
+                              "ligands": len(d)}
+    if a.serve_plain > 0:
+        _plain(ctx, reqs[:a.serve_plain], mids32, sp, ep, out, out["value"])
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -291,6 +418,9 @@ def main():
         sys.exit(2)
     if a.workload == "serve":
         serve(a)
+        return
+    if a.workload == "screen":
+        screen(a)
         return
     import adgpu
     lig, maps, tables, ep = build_inputs()

@@ -80,11 +80,13 @@ def test_device_test_records_match_oracle(gpu_ctx, c1g, nrun, pop, seed, kind):
 
 
 @pytest.mark.parametrize("ls,stopstd,min_frac", [("ad", 1.0, 0.0), ("sw", 0.5, 0.0), ("ad", 1.0, 0.3)])
-def test_run_stops_where_the_host_loop_stops(gpu_ctx, c1g, ls, stopstd, min_frac):
+def test_run_stops_where_the_host_loop_stops(gpu_ctx, c1g, ls, stopstd, min_frac, monkeypatch):
     """The device test inside adgpu_job_run against the reference's loop
     shape run on the host: every as_frequency generations download the
     energies, test them with the oracle, stop when the test passes and the
-    evals exceed min_as_evals."""
+    evals exceed min_as_evals.  Four run groups, so the test waits for every
+    group's generation and holds back their next ones."""
+    monkeypatch.setenv("ADGPU_RUN_GROUPS", "4")
     nrun, pop, asf, nev = 8, 60, 5, 400_000
     min_as = int(np.ceil(nev * min_frac)) * nrun
     a = _job(gpu_ctx, c1g, nrun, pop, ls=ls, max_evals=nev)

@@ -189,9 +189,9 @@ np.savez(sys.argv[1], pop=st["pop"], ene=st["ene"], ev=st["evals_runs"])
 
 @pytest.mark.parametrize("ls", ["ad", "sw", "fire"])
 def test_results_do_not_depend_on_launch_shape(gpu_ctx, c1g, ls, tmp_path):
-    """The persistent local searches in 64-thread workgroups, one run group,
-    or each group on the whole device give bit-identical jobs: an item's
-    result depends only on its own LCG streams."""
+    """Three run groups, two groups with 64-thread workgroups, one group, and
+    three groups each on the whole device give bit-identical jobs: an
+    item's result depends only on its own LCG streams."""
     import os
     import subprocess
     import sys
@@ -200,8 +200,9 @@ def test_results_do_not_depend_on_launch_shape(gpu_ctx, c1g, ls, tmp_path):
     paths = ", ".join(repr(p) for p in (os.path.join(repo, "autodock-gpu_amd"), os.path.join(repo, "oracle"), here))
     code = _SHAPE_CHILD.format(paths=paths, ls=ls)
     outs = []
-    for k, env in enumerate(({}, {"ADGPU_LS_BLOCK": "64"}, {"ADGPU_RUN_GROUPS": "1", "ADGPU_LS_BLOCK": "128"},
-                             {"ADGPU_LS_SHARE": "0"})):
+    for k, env in enumerate(({"ADGPU_RUN_GROUPS": "3"}, {"ADGPU_LS_BLOCK": "64", "ADGPU_RUN_GROUPS": "2"},
+                             {"ADGPU_RUN_GROUPS": "1", "ADGPU_LS_BLOCK": "128"},
+                             {"ADGPU_LS_SHARE": "0", "ADGPU_RUN_GROUPS": "3"})):
         f = os.path.join(str(tmp_path), "r%d.npz" % k)
         print("launch shape", env, flush=True)
         p = subprocess.run([sys.executable, "-c", code, f], env=dict(os.environ, **env), capture_output=True,
