@@ -68,6 +68,13 @@ class Context:
         most 32 atoms), 32 = two, 64 = one."""
         self._check(self.lib.adgpu_set_team_width(self.h, int(width)), "adgpu_set_team_width")
 
+    def set_team_table(self, table):
+        """Per-size team widths [(max_atoms, width), ...] ascending; [] clears."""
+        a = np.array([int(m) for m, _ in table] or [0], np.int32)
+        w = np.array([int(x) for _, x in table] or [0], np.int32)
+        self._check(self.lib.adgpu_set_team_table(self.h, len(table), a.ctypes.data_as(_ip), w.ctypes.data_as(_ip)),
+                    "adgpu_set_team_table")
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.adgpu_close(self.h)

@@ -984,9 +984,10 @@ struct Shape {
 
 constexpr size_t kLdsBudget = 64 * 1024;
 
-// Chooses the team width and atom chunks for a ligand: 32-lane teams (two
-// individuals per wavefront) up to 128 atoms, 64-lane teams beyond; 16-lane
-// teams (four per wavefront) on request for ligands of at most 32 atoms.
+// Chooses the team width and atom chunks for a ligand: the requested width,
+// else 32-lane teams (two individuals per wavefront) up to 128 atoms and
+// 64-lane teams beyond; 16-lane teams (four per wavefront) only for ligands
+// of at most 32 atoms.  Jobs pass the width fixed when they start.
 inline Shape choose_shape(const LigandDev& L, int team_width_hint, int threads = 256) {
     Shape s{};
     s.threads = threads;

@@ -50,6 +50,8 @@ struct adgpu_context {
     int next_id = 1;
     std::mutex mu;
     std::atomic<int> team_hint{0};   // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
+    std::mutex table_mu;
+    std::vector<std::pair<int, int>> team_table;   // (max atoms, width), ascending; adgpu_set_team_table
     // scratch for adgpu_eval_poses
     float* d_geno = nullptr;
     float* d_ene = nullptr;
@@ -96,6 +98,8 @@ struct adgpu_job {
     // serving slot: the ligand block belongs to the job (not a context id)
     bool own_lig = false;
     int want_groups = 1;                // run groups of a serving slot (fewer when a request has fewer runs)
+    bool shared = false;                // a serving slot: other requests share the device
+    int team_hint = 0;                  // team width of the job's kernels, fixed when it starts
     LigandSet own{};
     size_t own_cap = 0;
     float* d_best = nullptr;            // per-run best genotypes for FP64 re-scoring
@@ -136,16 +140,52 @@ struct adgpu_job {
 
 namespace {
 
+// The context's team width for a ligand of num_atoms atoms: its fixed width,
+// else its per-size table, else 0 (the kernels' built-in rule).
+int context_team_width(adgpu_context* c, int num_atoms) {
+    int w = c->team_hint.load(std::memory_order_relaxed);
+    if (w == 0) {
+        std::lock_guard<std::mutex> lk(c->table_mu);
+        for (const auto& e : c->team_table)
+            if (num_atoms <= e.first) { w = e.second; break; }
+    }
+    return w;
+}
+
+// team_hint < 0: the context's width for this ligand; else that width (a
+// job's, fixed when it starts).
 LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, hipStream_t stream = nullptr,
-                       int share = 1) {
+                       int share = 1, int team_hint = -1) {
     LaunchArgs la{};
     la.stream = stream ? stream : c->stream;
     la.share = share;
     la.lig_dev = reinterpret_cast<const LigandDev*>(ls.dev);
     la.lig_host = &ls.host;
     la.grid = ms.dev;
-    la.team_hint = c->team_hint.load(std::memory_order_relaxed);
+    la.team_hint = team_hint >= 0 ? team_hint : context_team_width(c, ls.host.num_atoms);
     return la;
+}
+
+// Local searches of one generation below which a job on its own leaves most
+// of the device idle: 4096 searches in 32-lane teams are 2048 wavefronts,
+// half of the 4096 wavefront slots of 256 compute units at 4 waves per SIMD.
+constexpr int64_t kFillSearches = 4096;
+
+// A job's team width, fixed when it starts: the context's width or table,
+// else 32 lanes up to 32 atoms and 64 above 128; between, 64 lanes (one atom
+// chunk, twice the wavefronts) when the job's own searches leave the device
+// mostly idle and nothing else shares it, else 32 lanes (the same work in
+// fewer instructions on a full device).  Measured, seconds per ligand or
+// served ligands/h: lognormal 8-90-atom mix, FIRE nrun 20 (3000 searches),
+// alone 0.101 s (32) vs 0.081 s (64); served by 32 slots 85.4 k/h (32) vs
+// 82.8 k/h (64 from 33 atoms); 10-60 atoms, ADAM nrun 50 (7500 searches),
+// alone 0.207 s (32) vs 0.220 s (64).
+int job_team_width(adgpu_context* c, int num_atoms, int64_t searches, bool shared) {
+    const int w = context_team_width(c, num_atoms);
+    if (w) return w;
+    if (num_atoms <= 32) return 32;
+    if (num_atoms > 128) return 64;
+    return (!shared && searches < kFillSearches) ? 64 : 32;
 }
 
 // Message of the calling thread's last failed call (jobs of one context may
@@ -337,6 +377,23 @@ int adgpu_set_team_width(adgpu_ctx c, int width) {
         return fail(c, ADGPU_ERR_ARG, "team width must be 0, 16, 32 or 64");
     std::lock_guard<std::mutex> lk(c->mu);
     c->team_hint = width;
+    return ADGPU_OK;
+}
+
+int adgpu_set_team_table(adgpu_ctx c, int n, const int32_t* max_atoms, const int32_t* widths) {
+    if (!c || n < 0 || n > 16 || (n > 0 && (!max_atoms || !widths)))
+        return fail(c, ADGPU_ERR_ARG, "team table: n must be 0..16 with both arrays");
+    std::vector<std::pair<int, int>> t;
+    for (int i = 0; i < n; ++i) {
+        const int a = max_atoms[i], w = widths[i];
+        if (a < 1 || a > ADGPU_MAX_ATOMS || (i > 0 && a <= max_atoms[i - 1]))
+            return fail(c, ADGPU_ERR_ARG, "team table: max_atoms must ascend within 1..256");
+        if (w != 16 && w != 32 && w != 64) return fail(c, ADGPU_ERR_ARG, "team table: widths are 16, 32 or 64");
+        if (w == 16 && a > 32) return fail(c, ADGPU_ERR_ARG, "team table: width 16 only up to 32 atoms");
+        t.emplace_back(a, w);
+    }
+    std::lock_guard<std::mutex> lk(c->table_mu);
+    c->team_table = std::move(t);
     return ADGPU_OK;
 }
 
@@ -663,11 +720,12 @@ static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire) {
 // Starts a search: uploads the initial population and PRNG seeds, clears
 // energies, eval counters, work queues and the stop flag, and resets the
 // generation state (the job's groups and buffers are already sized).
-static int job_start(adgpu_job* j, const adgpu_search_params* sp, int num_genes, const float* init_pop,
+static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDev& lig, const float* init_pop,
                      const uint32_t* seeds) {
     adgpu_context* c = j->ctx;
     j->sp = *sp;
-    j->sd = to_dev(sp, num_genes);
+    j->sd = to_dev(sp, lig.num_genes);
+    j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->shared);
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
     for (auto& g : j->groups) {
@@ -736,7 +794,7 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     rc = job_open(j.get());
     if (!rc) rc = job_set_groups(j.get(), ng, nrun);
     if (!rc) rc = job_reserve(j.get(), nrun, sp->pop_size, to_dev(sp, num_genes).num_ls, sp->ls_method == ADGPU_LS_FIRE);
-    if (!rc) rc = job_start(j.get(), sp, num_genes, init_pop, seeds);
+    if (!rc) rc = job_start(j.get(), sp, li->second.host, init_pop, seeds);
     if (rc) {
         const std::string msg = t_err;
         adgpu_job_destroy(j.release());
@@ -843,7 +901,7 @@ int adgpu_job_initpop(adgpu_job_t j) {
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = j->groups[0].stream;
     const JobBuffers b = slice(j, 0, j->sp.num_runs, whole_parity(j), 0);
-    HIPCHK(c, launch_initpop(launch_args(c, *ls, *ms, s), j->sd, b));
+    HIPCHK(c, launch_initpop(launch_args(c, *ls, *ms, s, 1, j->team_hint), j->sd, b));
     HIPCHK(c, launch_sum_evals(s, j->sd, b));
     HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;
@@ -857,7 +915,8 @@ int adgpu_job_kernel_gen(adgpu_job_t j) {
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = j->groups[0].stream;
-    HIPCHK(c, launch_gen(launch_args(c, *ls, *ms, s), j->sd, slice(j, 0, j->sp.num_runs, whole_parity(j), 0)));
+    HIPCHK(c, launch_gen(launch_args(c, *ls, *ms, s, 1, j->team_hint), j->sd,
+                         slice(j, 0, j->sp.num_runs, whole_parity(j), 0)));
     HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;
 }
@@ -872,7 +931,8 @@ int adgpu_job_kernel_ls(adgpu_job_t j, int generation) {
     const int m = ls_method_for(j, generation);
     if (m == ADGPU_LS_NONE) return ADGPU_OK;
     hipStream_t s = j->groups[0].stream;
-    HIPCHK(c, launch_ls(launch_args(c, *ls, *ms, s), j->sd, slice(j, 0, j->sp.num_runs, whole_parity(j), 0), m));
+    HIPCHK(c, launch_ls(launch_args(c, *ls, *ms, s, 1, j->team_hint), j->sd,
+                        slice(j, 0, j->sp.num_runs, whole_parity(j), 0), m));
     HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;
 }
@@ -903,7 +963,7 @@ static int enqueue_generation(adgpu_job_t j, int k, const LigandSet* ls, const M
     adgpu_context* c = j->ctx;
     RunGroup& g = j->groups[k];
     const int slot = (int)(g.enq % kRing);
-    const LaunchArgs la = launch_args(c, *ls, *ms, g.stream, (int)j->groups.size());
+    const LaunchArgs la = launch_args(c, *ls, *ms, g.stream, (int)j->groups.size(), j->team_hint);
     const SearchDev sd = slice_sd(j, g.r0, g.r1);
     const JobBuffers b = slice(j, g.r0, g.r1, g.parity, k);
     const int m = ls_method_for(j, j->generation + g.enq);
@@ -1361,6 +1421,14 @@ int check_request(adgpu_context* c, int map_id, const PreparedLigand& lig, const
     return ADGPU_OK;
 }
 
+int map_hold(adgpu_context* c, int map_id, int delta) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto mi = c->maps.find(map_id);
+    if (mi == c->maps.end()) return fail(c, ADGPU_ERR_ARG, "unknown map id");
+    mi->second.jobs += delta;
+    return ADGPU_OK;
+}
+
 size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params* sp, int as_frequency) {
     const size_t nrun = (size_t)sp->num_runs, ne = nrun * (size_t)sp->pop_size;
     size_t b = 2 * ne * kGenes * sizeof(float) + 2 * ne * sizeof(float) + ne * sizeof(int32_t) +
@@ -1379,12 +1447,13 @@ size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params
     return b;
 }
 
-int slot_open(adgpu_context* c, int run_groups, adgpu_job** out) {
+int slot_open(adgpu_context* c, int run_groups, bool shared, adgpu_job** out) {
     *out = nullptr;
     HIPCHK(c, hipSetDevice(c->device));
     std::unique_ptr<adgpu_job> j(new adgpu_job());
     j->ctx = c;
     j->own_lig = true;
+    j->shared = shared;
     j->want_groups = std::max(1, std::min(run_groups, kMaxGroups));
     int rc = job_open(j.get());
     if (!rc) rc = job_set_groups(j.get(), run_groups, run_groups);
@@ -1430,7 +1499,7 @@ int slot_bind(adgpu_job* j, int map_id, const PreparedLigand& lig, const adgpu_s
                                             j->groups[0].stream);
         if (e != hipSuccess) rc = fail(c, ADGPU_ERR_HIP, std::string("hipMemcpyAsync(ligand): ") + hipGetErrorString(e));
     }
-    if (!rc) rc = job_start(j, sp, lig.hd.num_genes, init_pop, seeds);   // waits for the copies
+    if (!rc) rc = job_start(j, sp, lig.hd, init_pop, seeds);   // waits for the copies
     if (rc) {
         const std::string msg = t_err;
         slot_unbind(j);
@@ -1480,7 +1549,7 @@ int slot_results(adgpu_job* j, float* best_geno, float* best_e, double* best_f64
     }
     hipStream_t s = j->groups[0].stream;
     HIPCHK(c, hipMemcpyAsync(j->d_best, geno.data(), geno.size() * sizeof(float), hipMemcpyHostToDevice, s));
-    HIPCHK(c, launch_rescore_f64(launch_args(c, *ls, *ms, s), j->d_best, nrun, j->d_f64));
+    HIPCHK(c, launch_rescore_f64(launch_args(c, *ls, *ms, s, 1, j->team_hint), j->d_best, nrun, j->d_f64));
     HIPCHK(c, hipMemcpyAsync(best_f64, j->d_f64, (size_t)nrun * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     return ADGPU_OK;

@@ -127,6 +127,7 @@ void worker(adgpu_server* s, int slot) {
         }
         s->cv_work.notify_one();   // the next request may fit beside this one
         run_request(s, j, r);
+        (void)map_hold(s->ctx, r->map_id, -1);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->in_use -= r->bytes;
@@ -174,7 +175,7 @@ int adgpu_server_create(adgpu_ctx ctx, const adgpu_server_config* cfg, adgpu_ser
     s->t0 = Clock::now();
     for (int k = 0; k < cfg->slots; ++k) {
         adgpu_job* j = nullptr;
-        int rc = slot_open(ctx, cfg->run_groups > 0 ? cfg->run_groups : 1, &j);
+        int rc = slot_open(ctx, cfg->run_groups > 0 ? cfg->run_groups : 1, cfg->slots > 1, &j);
         if (rc) {
             const std::string msg = thread_error();
             for (adgpu_job* o : s->slots) adgpu_job_destroy(o);
@@ -212,9 +213,13 @@ int adgpu_server_submit(adgpu_server_t s, const adgpu_request* q, int64_t* ticke
     r->res.device_bytes = (int64_t)r->bytes;
     r->res.slot = -1;
     if (r->bytes > s->budget) return set_error(ADGPU_ERR_LIMIT, "request larger than the server's memory budget");
+    if ((rc = map_hold(s->ctx, r->map_id, +1))) return rc;   // released when the request ends
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stopping) return set_error(ADGPU_ERR_STATE, "server is shutting down");
+        if (s->stopping) {
+            (void)map_hold(s->ctx, r->map_id, -1);
+            return set_error(ADGPU_ERR_STATE, "server is shutting down");
+        }
         r->ticket = s->next_ticket++;
         r->res.t_submit_ms = s->now_ms();
         *ticket = r->ticket;
@@ -289,6 +294,7 @@ int adgpu_server_destroy(adgpu_server_t s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
+        for (Request* r : s->queue) (void)map_hold(s->ctx, r->map_id, -1);   // dropped unrun
         s->queue.clear();
     }
     s->cv_work.notify_all();

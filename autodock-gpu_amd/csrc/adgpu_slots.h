@@ -29,12 +29,17 @@ int prepare_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_p
 // Checks a request against the context: the map set exists and has the
 // ligand's type maps, and the search parameters are valid.
 int check_request(adgpu_context* c, int map_id, const PreparedLigand& lig, const adgpu_search_params* sp);
+// Adds delta to the uses of a map set (a queued or running request holds
+// one, so the map set cannot be released under it).
+int map_hold(adgpu_context* c, int map_id, int delta);
 // Device bytes one request occupies in a slot (population buffers, PRNG
 // states, ligand block, AutoStop state when as_frequency > 0, re-scoring
 // scratch).
 size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params* sp, int as_frequency);
 
-int slot_open(adgpu_context* c, int run_groups, adgpu_job** out);
+// shared: other slots' requests run on the device at the same time (the
+// job's team width then assumes a full device).
+int slot_open(adgpu_context* c, int run_groups, bool shared, adgpu_job** out);
 // Binds the slot to a request: holds a use of the map set, grows buffers
 // that are too small, uploads the ligand block, the initial population and
 // the PRNG seeds, and resets the generation state.

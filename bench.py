@@ -68,6 +68,10 @@ def parse():
     ap.add_argument("--hw-queues", type=int, default=8, choices=range(1, 33), metavar="1..32")
     ap.add_argument("--workload", default="lga", choices=["lga", "serve", "screen"])
     ap.add_argument("--screen-ligands", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true",
+                    help="serve/screen: measure the team width per ligand size first (adgpu.autotune)")
+    ap.add_argument("--team-table", default="",
+                    help="serve/screen: fixed team widths per ligand size, e.g. 32:32,128:64 (max atoms:width)")
     ap.add_argument("--serve-ligands", type=int, default=300)
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-evals", type=int, default=1000000)
@@ -247,8 +251,9 @@ def _saturation(ctx, slots, reqs, mids, sp, ep, label):
 
 def _plain(ctx, reqs, mids32, sp, ep, out, served_value):
     """The plain form: FP32 maps, one ligand at a time through the job API,
-    with the default run groups and with one stream (the reference's loop
-    structure); per-ligand FP64 best energies."""
+    with the default run groups and team widths, and with one stream (the
+    reference's loop structure); per-ligand FP64 best energies."""
+    ctx.set_team_table([])
     for label, groups in (("plain_f32_one_at_a_time", None), ("plain_f32_one_stream", "1")):
         if groups is None:
             os.environ.pop("ADGPU_RUN_GROUPS", None)
@@ -289,6 +294,26 @@ def _best_on_f32(ctx, reqs, res, mids32, ep):
     return best
 
 
+def _autotune(ctx, a, ls):
+    """--team-table, or with --autotune the team-width table measured for
+    this local search with the device shared by many requests
+    (adgpu.autotune, mode "served"), installed in the context for the served
+    phases; else None (built-in rule)."""
+    if a.team_table:
+        table = [tuple(int(x) for x in e.split(":")) for e in a.team_table.split(",")]
+        ctx.set_team_table(table)
+        return {"table": table, "seconds": 0.0, "evals_per_s": None}
+    if not a.autotune:
+        return None
+    from adgpu import autotune
+    t0 = time.perf_counter()
+    table, rates = autotune.tune(ctx, ls=ls, mode="served", nrun=20, evals=100000, reps=1)
+    dt = time.perf_counter() - t0
+    print("autotune (%s, %.1f s): %s" % (ls, dt, table), file=sys.stderr, flush=True)
+    return {"table": table, "seconds": dt,
+            "evals_per_s": {str(n): {str(w): v for w, v in r.items()} for n, r in rates.items()}}
+
+
 def serve(a):
     """The serving-loop measurement (module docstring)."""
     from adgpu._abi import energy_params
@@ -302,6 +327,7 @@ def serve(a):
                              spacing=0.375)
     mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
     phases = set(a.serve_phases.split(","))
+    tuned = _autotune(ctx, a, "fire")
     out = {
         "metric": "ligands/hour of the serving loop (FP16 maps, concurrent slots); p50/p99 ligand latency",
         "value": None,
@@ -315,6 +341,7 @@ def serve(a):
                                % (len(reqs), a.serve_nrun, a.serve_evals, a.pop),
                    "slots": a.serve_slots, "hw_queues": a.hw_queues,
                    "host_wait": os.environ.get("ADGPU_HOST_WAIT")},
+        "team_table": tuned,
     }
     _run_served(ctx, a.serve_slots, reqs[:a.serve_slots], mids16, sp, ep)   # warm-up
     res = []
@@ -375,7 +402,9 @@ def screen(a):
     sp = adgpu.search_params(num_runs=50, pop_size=a.pop, max_evals=a.evals, ls="adam", spacing=0.375)
     mids32 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F32) for m in map_sets]
     mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
+    tuned = _autotune(ctx, a, "adam")
     out = {
+        "team_table": tuned,
         "metric": "ligands/hour of a batched screen against one receptor",
         "value": None,
         "unit": "ligands/h",

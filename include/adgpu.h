@@ -162,12 +162,24 @@ int  adgpu_close(adgpu_ctx ctx);
 const char* adgpu_last_error(adgpu_ctx ctx);
 /* Version string; also usable without a GPU. */
 const char* adgpu_version(void);
-/* Lanes that evaluate one individual: 0 = chosen per ligand (32 up to 128
- * atoms, 64 above), 16 (four individuals per wavefront, ligands of at most 32
- * atoms; wider teams otherwise), 32 (two individuals per wavefront) or 64.  Plays the role
- * of the reference's build-time NUM_OF_THREADS_PER_BLOCK (common/defines.h:29-49)
- * as a run-time choice; results agree within the FP32 tolerance either way. */
+/* Lanes that evaluate one individual: 0 = chosen per ligand (the team table
+ * below if set; else for a job 32 up to 32 atoms, 64 above 128 atoms, and
+ * between them 64 when the job's own local searches of a generation (fewer
+ * than 4096) leave the device mostly idle and no other request shares it,
+ * else 32; for per-pose calls 32 up to 128 atoms), 16 (four individuals per
+ * wavefront, ligands of at most 32 atoms; wider teams otherwise), 32 (two
+ * individuals per wavefront) or 64.  Plays the role of the reference's
+ * build-time NUM_OF_THREADS_PER_BLOCK (common/defines.h:29-49) as a run-time
+ * choice; results agree within the FP32 tolerance either way. */
 int  adgpu_set_team_width(adgpu_ctx ctx, int width);
+/* Team width per ligand size, used when no width is set: a ligand of N atoms
+ * takes widths[i] of the first entry with N <= max_atoms[i] (ascending; 16,
+ * 32 or 64, 16 only up to 32 atoms); larger ligands keep the built-in rule.
+ * n = 0 clears the table.  Like the width itself it stands for the
+ * reference's build-time NUM_OF_THREADS_PER_BLOCK (common/defines.h:29-49);
+ * adgpu.autotune fills it from measured local-search launches.  Set it before
+ * creating the jobs that should use it. */
+int  adgpu_set_team_table(adgpu_ctx ctx, int n, const int32_t* max_atoms, const int32_t* widths);
 
 /* --- resident inputs ---------------------------------------------------------
  * Replace the per-job uploads in docking_with_gpu

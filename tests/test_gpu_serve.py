@@ -2,20 +2,21 @@
 
 * Each request's result is the same alone and in the mix.  Uneven ligands
   (10-90 atoms, 0-25 torsions) against three resident map sets (FP32 and
-  FP16), all four local searches, one request with AutoStop, are docked
-  one at a time through the plain job API, then all at once through a
-  server whose slots run them concurrently.  Per-run best genotypes and
-  energies, generation counts and eval totals must be bit-identical: every
-  search is seeded by its own request, and nothing of one request reaches
-  another's kernels.
+  FP16), all four local searches, one request with AutoStop, are served one
+  at a time (nothing else on the device), then all at once by the same
+  server configuration, whose slots run them concurrently.  Per-run best
+  genotypes and energies, generation counts and eval totals must be
+  bit-identical: every search is seeded by its own request, and nothing of
+  one request reaches another's kernels.  With the context's team width
+  fixed, served results also equal the plain job API's.
 * Admission by memory: a budget that holds only a few requests keeps the
   bytes in use under it at every moment, and the results are again the same.
 * The per-run best energies equal the oracle's FP32 energy of the returned
   genotypes (the bar of helpers.check_energies), and the FP64 re-scoring of
   each run's best pose equals the FP64 oracle to 1e-10 of the sum of |terms|.
 * Refused at submit: a request larger than the whole budget, an unknown map
-  set; a map set in use by a queued or running request cannot be released
-  while that request runs.
+  set.  A map set cannot be released while a queued or running request
+  uses it.
 """
 import numpy as np
 import pytest
@@ -69,7 +70,8 @@ def mix(gpu_ctx):
         init = O.initial_population((k, 17), NRUN, POP, maps.size_angstr, maps.spacing)
         seeds = O.cmwc_uints((k, 17), NRUN * POP * 64)
         autostop = (0.15, 5, 0) if k == 3 else None
-        reqs.append(dict(k=k, mid=mid, T=T, sp=sp, init=init, seeds=seeds, autostop=autostop, cg=cg, ep=ep))
+        reqs.append(dict(k=k, mid=mid, T=T, sp=sp, init=init, seeds=seeds, autostop=autostop, cg=cg, ep=ep,
+                         types=types))
     yield reqs
     for _, mid, _ in sets:
         gpu_ctx.maps_release(mid)
@@ -90,12 +92,20 @@ def _alone(ctx, q):
                 best_f64=np.stack([b["total"], b["inter"], b["intra"]], axis=1))
 
 
-def _serve(ctx, reqs, slots, budget=0):
+def _serve(ctx, reqs, slots, budget=0, one_at_a_time=False):
+    """Results of the requests through one server; one_at_a_time: each
+    request finishes before the next is submitted (nothing else on the
+    device)."""
     with adgpu.Server(ctx, slots=slots, memory_budget=budget) as srv:
-        tickets = [srv.submit(q["mid"], q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"], autostop=q["autostop"],
-                              rescore_f64=True) for q in reqs]
-        print("submitted", len(tickets), "slots", slots, "budget", budget, flush=True)
-        out = [srv.result(t) for t in tickets]
+        def sub(q):
+            return srv.submit(q["mid"], q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"], autostop=q["autostop"],
+                              rescore_f64=True)
+        if one_at_a_time:
+            out = [srv.result(sub(q)) for q in reqs]
+        else:
+            tickets = [sub(q) for q in reqs]
+            print("submitted", len(tickets), "slots", slots, "budget", budget, flush=True)
+            out = [srv.result(t) for t in tickets]
         stats = srv.stats()
     return out, stats
 
@@ -109,10 +119,7 @@ def _same(a, b, k):
 
 
 def test_each_result_is_the_same_alone_and_in_the_mix(gpu_ctx, mix):
-    alone = []
-    for q in mix:
-        print("alone", q["k"], "atoms", q["T"].num_atoms, "genes", q["T"].num_rotbonds + 6, flush=True)
-        alone.append(_alone(gpu_ctx, q))
+    alone, _ = _serve(gpu_ctx, mix, slots=5, one_at_a_time=True)
     served, stats = _serve(gpu_ctx, mix, slots=5)
     print("stats", stats, flush=True)
     assert stats["completed"] == len(mix) and stats["failed"] == 0
@@ -121,12 +128,32 @@ def test_each_result_is_the_same_alone_and_in_the_mix(gpu_ctx, mix):
         assert s["status"] == 0, s["message"]
         assert s["t_submit_ms"] <= s["t_admit_ms"] <= s["t_done_ms"]
         _same(a, s, q["k"])
-    # the AutoStop request ran under AutoStop: at most the generations of the plain search
+    # the AutoStop request either used its whole budget or stopped at a test generation
     assert served[3]["total_evals"] >= NRUN * 12000 or served[3]["generations"] % 5 == 0
 
 
+def test_served_results_equal_the_job_api(gpu_ctx, mix):
+    """With the team width fixed for the context, a served request and the
+    same docking through the job API (its own ligand upload, job, run, FP64
+    re-scoring) are bit-identical.  Without a fixed width a server that
+    shares the device between requests may pick 32-lane teams where a job on
+    its own picks 64 (adgpu_set_team_width)."""
+    gpu_ctx.set_team_width(32)
+    try:
+        alone = []
+        for q in mix:
+            print("job api", q["k"], "atoms", q["T"].num_atoms, "genes", q["T"].num_rotbonds + 6, flush=True)
+            alone.append(_alone(gpu_ctx, q))
+        served, _ = _serve(gpu_ctx, mix, slots=4)
+    finally:
+        gpu_ctx.set_team_width(0)
+    for q, a, s in zip(mix, alone, served):
+        assert s["status"] == 0, s["message"]
+        _same(a, s, q["k"])
+
+
 def test_admission_by_memory_keeps_bytes_under_the_budget(gpu_ctx, mix):
-    ref, _ = _serve(gpu_ctx, mix, slots=1)
+    ref, _ = _serve(gpu_ctx, mix, slots=3)
     sizes = sorted(r["device_bytes"] for r in ref)
     budget = sizes[-1] + sizes[-2]            # any two requests fit, not every three
     served, stats = _serve(gpu_ctx, mix, slots=8, budget=budget)
@@ -166,3 +193,15 @@ def test_refusals(gpu_ctx, mix):
         assert r["status"] == 0
         assert srv.next_done(0) is None       # result() already took it
         assert srv.stats()["completed"] == 1
+
+
+def test_map_set_is_held_by_queued_and_running_requests(gpu_ctx, mix):
+    q = mix[1]
+    maps = synth.make_maps(q["types"], npts=40, seed=77)
+    mid = gpu_ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing)
+    with adgpu.Server(gpu_ctx, slots=1) as srv:
+        t = [srv.submit(mid, q["T"], q["sp"], q["init"], q["seeds"], ep=q["ep"]) for _ in range(2)]
+        with pytest.raises(adgpu.AdgpuError, match="still used"):
+            gpu_ctx.maps_release(mid)          # one request running, one queued
+        assert all(srv.result(x)["status"] == 0 for x in t)
+    gpu_ctx.maps_release(mid)                  # both finished: the map set can go

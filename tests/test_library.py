@@ -43,3 +43,12 @@ def test_oracle_symbols():
     lib = oracle.lib()
     names = _declared(os.path.join(REPO, "oracle", "adock_oracle.h"))
     assert not [n for n in names if not hasattr(lib, n)]
+
+
+def test_autotune_table_from_rates():
+    """Classes bounded by the measured sizes; equal neighbours merge."""
+    from adgpu.autotune import table_from
+    rates = {24: {16: 1.0, 32: 3.0, 64: 2.0}, 32: {16: 1.0, 32: 3.0, 64: 2.0}, 40: {32: 1.0, 64: 2.0},
+             64: {32: 1.0, 64: 2.0}, 96: {32: 5.0, 64: 2.0}}
+    assert table_from(rates) == [(32, 32), (64, 64), (96, 32)]
+    assert table_from({40: {32: 2.0, 64: 1.0}}) == [(40, 32)]
