@@ -13,7 +13,12 @@ Prints one JSON line: evals/s (value), ligands/hour, p50/p99 per-ligand
 latency, the roofline of the dominant kernel (the local-search launch) and a
 CPU baseline (the oracle's LGA on the host cores, bounded sample).
 
---workload screen measures a batched screen (seeded ligands of 10-60 atoms
+--workload poses evaluates 1e5 seeded random poses of the same ligand (energy
+and gradient) and compares each FP32 energy with the FP64 re-scoring of the
+pose on the GPU, by pose class.  --workload flexres docks the ligand plus two
+flexible residues (tables built by the reference's host code, kept in
+tests/golden/ref_flex.npz) on 126^3 BF16 maps with ADADELTA, nrun 100, and
+reports the same per-pose comparison.  --workload screen measures a batched screen (seeded ligands of 10-60 atoms
 and 0-20 torsions, one 80^3 map set, ADAM, nrun 50) through the same server
 with FP32 and FP16 maps.  --workload serve measures the serving loop instead: seeded ligands of uneven
 size (atoms lognormal with median 35 and at most 90, torsions 0-25) docked
@@ -66,7 +71,8 @@ def parse():
     # default is 4): with 8 the library runs the job's runs in four groups on
     # four streams, each on its own queue
     ap.add_argument("--hw-queues", type=int, default=8, choices=range(1, 33), metavar="1..32")
-    ap.add_argument("--workload", default="lga", choices=["lga", "serve", "screen"])
+    ap.add_argument("--workload", default="lga", choices=["lga", "serve", "screen", "poses", "flexres"])
+    ap.add_argument("--poses", type=int, default=100000)
     ap.add_argument("--screen-ligands", type=int, default=64)
     ap.add_argument("--autotune", action="store_true",
                     help="serve/screen: measure the team width per ligand size first (adgpu.autotune)")
@@ -159,6 +165,106 @@ def cpu_baseline(maps, tables, ep, sp_gpu, seconds):
     return dict(value=evals / dt, unit="evals/s", cores=threads, kind="port",
                 sample="%d runs x pop %d, %d LGA generations with ADADELTA (%d evals, %.1f s)"
                        % (nrun, sp.pop_size, gens, evals, dt))
+
+
+def _pose_errors(ctx, mid, lid, poses):
+    """FP32 energies and gradients of the poses (the kernels' arithmetic,
+    adgpu_eval_poses) against the FP64 energies of the same poses
+    (adgpu_rescore_f64), on the GPU.  Classes: out of the grid (some atom
+    outside the maps: +2^24 each), clash (|E| > 1000 kcal/mol), normal.
+    Error = |E32 - E64| / max(1, |E64|)."""
+    import adgpu
+    t0 = time.perf_counter()
+    e32, _ = ctx.eval_poses(mid, lid, poses, adgpu.EVAL_ENERGRAD)
+    dt = time.perf_counter() - t0
+    tot, _, _ = ctx.rescore_f64(mid, lid, poses)
+    rel = np.abs(e32.astype(np.float64) - tot) / np.maximum(1.0, np.abs(tot))
+    out_grid = tot >= 0.5 * 2.0 ** 24
+    clash = ~out_grid & (np.abs(tot) > 1000.0)
+    normal = ~out_grid & ~clash
+    res = {"poses": len(poses), "energrad_poses_per_s": len(poses) / dt,
+           "energrad_note": "host buffers in and out (PCIe included)"}
+    for name, m in (("out_of_grid", out_grid), ("clash", clash), ("normal", normal)):
+        r = rel[m]
+        res[name] = {"fraction": float(np.mean(m)),
+                     "rel_err_median": float(np.median(r)) if r.size else None,
+                     "rel_err_p99": float(np.percentile(r, 99)) if r.size else None,
+                     "rel_err_max": float(np.max(r)) if r.size else None}
+    return res
+
+
+def poses(a):
+    """Per-pose measurement on the LGA workload's ligand and maps."""
+    import adgpu
+    from adgpu import synth
+    lig, maps, tables, ep = build_inputs()
+    ctx = adgpu.Context(0)
+    mid = ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing, adgpu.MAP_F32)
+    lid = ctx.ligand_upload(tables, ep)
+    ps = synth.random_poses(a.poses, tables.num_genes, maps.size, seed=2024)
+    ctx.eval_poses(mid, lid, ps[:1000], adgpu.EVAL_ENERGRAD)   # warm-up
+    out = {"metric": "per-pose FP32 energy+gradient against FP64 re-scoring", "unit": "poses/s",
+           "config": {"workload": "%d seeded random poses, 30 atoms / 6 torsions, 60^3 FP32 maps" % a.poses},
+           "data": "synthetic (seeded ligand, maps and poses)"}
+    out.update(_pose_errors(ctx, mid, lid, ps))
+    out["value"] = out["energrad_poses_per_s"]
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def _flex_tables():
+    """Ligand (30 atoms / 6 torsions) plus two flexible residues (12 atoms /
+    4 torsions each), tables as the reference's host code builds them, from
+    tests/golden/ref_flex.npz; returns (tables, map types, energy params)."""
+    from types import SimpleNamespace
+    from adgpu._abi import energy_params
+    f = np.load(os.path.join(HERE, "tests", "golden", "ref_flex.npz"))
+    keys = ("charges", "atom_types", "atom_map", "ignore_inter", "ref_coords", "rotbond_moving", "rotbond_unit",
+            "rotbond_atoms", "atom_rotbonds", "pairs", "atype_reqm", "vw_exp", "reqm_ab", "vw_ac", "vw_bd", "ds_s",
+            "ds_v")
+    t = SimpleNamespace(**{k: f[k] for k in keys})
+    t.num_atoms, t.true_ligand_atoms = int(f["num_atoms"]), int(f["true_ligand_atoms"])
+    t.num_rotbonds, t.num_atypes = int(f["num_rotbonds"]), int(f["num_atypes"])
+    t.num_genes = t.num_rotbonds + 6
+    ep = energy_params(coeff_elec=float(f["coeff_elec"]), coeff_desolv=float(f["coeff_desolv"]), qasp=float(f["qasp"]),
+                       smooth=float(f["smooth"]), elec_min_distance=float(f["elec_min"]))
+    return t, [str(x) for x in f["map_types"]], ep
+
+
+def flexres(a):
+    """Flexible-receptor docking: ligand + two flexible residues on 126^3
+    BF16 maps, ADADELTA, nrun 100; throughput and per-pose FP32 vs FP64."""
+    import adgpu
+    from adgpu import synth
+    tables, types, ep = _flex_tables()
+    maps = synth.make_maps(types, npts=126, spacing=0.375, seed=11)
+    ctx = adgpu.Context(0)
+    mid = ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing, adgpu.MAP_BF16)
+    lid = ctx.ligand_upload(tables, ep)
+    sp = adgpu.search_params(num_runs=a.nrun, pop_size=a.pop, max_evals=a.evals, ls="ad", spacing=maps.spacing)
+    lat, evals = [], []
+    for step in range(a.warmup + a.steps):
+        init, seeds = initial_state(3000 + step, a.nrun, a.pop, maps)
+        job = ctx.job(mid, lid, sp, init, seeds)
+        t0 = time.perf_counter()
+        job.initpop()
+        gens, total = job.run()
+        dt = time.perf_counter() - t0
+        job.destroy()
+        print("step %d: %.3f s, %d generations, %d evals" % (step, dt, gens, total), file=sys.stderr, flush=True)
+        if step >= a.warmup:
+            lat.append(dt)
+            evals.append(total)
+    out = {"metric": "LGA energy evals/sec, ligand + 2 flexible residues", "value": float(np.sum(evals) / np.sum(lat)),
+           "unit": "evals/s", "ligands_per_hour": 3600.0 / float(np.mean(lat)),
+           "config": {"workload": "ligand 30 atoms / 6 torsions + 2 flexible residues (12 atoms / 4 torsions), "
+                                  "126^3 BF16 maps at 0.375 A, ADADELTA, nrun %d, pop %d, %d evals per run"
+                                  % (a.nrun, a.pop, a.evals)},
+           "data": "synthetic (seeded maps; reference-built ligand and residue tables)"}
+    ps = synth.random_poses(10000, tables.num_genes, maps.size, seed=77)
+    out["per_pose"] = _pose_errors(ctx, mid, lid, ps)
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def _lognormal_ligand(k):
@@ -450,6 +556,12 @@ def main():
         return
     if a.workload == "screen":
         screen(a)
+        return
+    if a.workload == "poses":
+        poses(a)
+        return
+    if a.workload == "flexres":
+        flexres(a)
         return
     import adgpu
     lig, maps, tables, ep = build_inputs()
