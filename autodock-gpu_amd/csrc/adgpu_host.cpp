@@ -94,7 +94,7 @@ struct adgpu_job {
     int cap_runs = 0;
     size_t as_bins_cap = 0;
     int as_log_alloc = 0;
-    int64_t reallocs = 0;               // buffer growths after the first allocation
+    std::atomic<int64_t> reallocs{0};   // buffer growths after the first allocation (read by server stats)
     // serving slot: the ligand block belongs to the job (not a context id)
     bool own_lig = false;
     int want_groups = 1;                // run groups of a serving slot (fewer when a request has fewer runs)

@@ -43,6 +43,7 @@ struct Request {
     int flags = 0;
     size_t bytes = 0;
     bool done = false;
+    bool claimed = false;   // a thread is in adgpu_server_result for this ticket
     adgpu_request_result res{};
     std::vector<float> best_geno, best_e, pop, ene;
     std::vector<double> best_f64;
@@ -249,9 +250,14 @@ int adgpu_server_result(adgpu_server_t s, int64_t ticket, int timeout_ms, adgpu_
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return set_error(ADGPU_ERR_ARG, "unknown ticket");
     Request* r = it->second.get();
-    if (!wait_for(s->cv_done, lk, timeout_ms, [&] { return r->done; }))
+    if (r->claimed) return set_error(ADGPU_ERR_STATE, "another thread is collecting this ticket");
+    r->claimed = true;
+    if (!wait_for(s->cv_done, lk, timeout_ms, [&] { return r->done; })) {
+        r->claimed = false;
         return set_error(ADGPU_ERR_TIMEOUT, "request not finished within the timeout");
-    if (best_f64 && r->res.status == ADGPU_OK && r->best_f64.empty())
+    }
+    r->claimed = false;
+    if (best_f64 && r->res.status == ADGPU_OK && r->best_f64.empty())   // the ticket stays collectable
         return set_error(ADGPU_ERR_STATE, "request ran without ADGPU_REQ_RESCORE_F64");
     if ((population || energies) && r->res.status == ADGPU_OK && r->pop.empty())
         return set_error(ADGPU_ERR_STATE, "request ran without ADGPU_REQ_KEEP_POPULATION");

@@ -378,7 +378,8 @@ int  adgpu_server_submit(adgpu_server_t srv, const adgpu_request* req, int64_t* 
  * (< 0: no limit) and returns ADGPU_ERR_TIMEOUT when none finished. */
 int  adgpu_server_next_done(adgpu_server_t srv, int timeout_ms, int64_t* ticket);
 /* Waits up to timeout_ms (< 0: no limit) for the request, copies its result
- * and forgets the ticket.  best_genotypes: num_runs*64 floats, best_energies:
+ * and forgets the ticket (one thread at a time per ticket; a second one gets
+ * ADGPU_ERR_STATE).  best_genotypes: num_runs*64 floats, best_energies:
  * num_runs, best_f64: 3*num_runs doubles (total, inter, intra; needs
  * ADGPU_REQ_RESCORE_F64), population: num_runs*pop_size*64 and energies:
  * num_runs*pop_size (need ADGPU_REQ_KEEP_POPULATION).  Any pointer may be

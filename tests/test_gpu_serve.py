@@ -205,3 +205,23 @@ def test_map_set_is_held_by_queued_and_running_requests(gpu_ctx, mix):
             gpu_ctx.maps_release(mid)          # one request running, one queued
         assert all(srv.result(x)["status"] == 0 for x in t)
     gpu_ctx.maps_release(mid)                  # both finished: the map set can go
+
+
+def test_one_collector_per_ticket(gpu_ctx, mix):
+    """While one thread waits for a ticket, another thread's call for the
+    same ticket is refused (ADGPU_ERR_STATE); the first still gets the result."""
+    import threading
+    import time
+    q = mix[1]
+    sp = search_params(num_runs=NRUN, pop_size=POP, max_evals=3000000, ls="ad", spacing=0.375)
+    with adgpu.Server(gpu_ctx, slots=1) as srv:
+        t = srv.submit(q["mid"], q["T"], sp, q["init"], q["seeds"], ep=q["ep"])
+        got = {}
+        th = threading.Thread(target=lambda: got.setdefault("r", srv.result(t)))
+        th.start()
+        time.sleep(0.05)
+        if th.is_alive():                      # the first collector is waiting
+            with pytest.raises(adgpu.AdgpuError, match="another thread"):
+                srv.result(t, timeout_ms=0)
+        th.join(timeout=120)
+        assert not th.is_alive() and got["r"]["status"] == 0
