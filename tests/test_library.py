@@ -36,6 +36,16 @@ def test_null_arguments_are_rejected_without_gpu():
     assert lib.adgpu_close(None) == 1
     assert lib.adgpu_job_destroy(None) == 1
     assert lib.adgpu_last_error(None) == b"null context"
+    # serving loop and team table: argument checks come before any HIP call
+    cfg = _abi.ServerConfig(4, 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.adgpu_server_create(None, ctypes.byref(cfg), ctypes.byref(h)) == 1
+    assert lib.adgpu_server_submit(None, None, None) == 1
+    assert lib.adgpu_server_next_done(None, 0, None) == 1
+    assert lib.adgpu_server_result(None, 1, 0, None, None, None, None, None, None) == 1
+    assert lib.adgpu_server_stats_get(None, None) == 1
+    assert lib.adgpu_server_destroy(None) == 1
+    assert lib.adgpu_set_team_table(None, 0, None, None) == 1
 
 
 def test_oracle_symbols():
