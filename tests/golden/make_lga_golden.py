@@ -1,0 +1,69 @@
+"""Writes tests/golden/lga_c1_oracle.npz: the oracle's whole LGA (the FP32
+restatement of the reference's device control flow, oracle/) on the
+30-atom / 6-torsion ligand (seed 7) and 60^3 maps (seed 11), ADADELTA,
+nrun 100, pop 150, 2.5 M evaluations per run, from fixed initial population
+and LCG seeds: each run's best energy and genotype, the generation count and
+the eval totals.  The oracle needs minutes of CPU for this, so the GPU test
+(tests/test_gpu_lga.py) compares the device's run of the same job with the
+stored result.  Deterministic: the oracle's runs are independent, so the
+thread count does not change the result.
+
+Usage: python tests/golden/make_lga_golden.py [threads]
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "autodock-gpu_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+NRUN, POP, EVALS, SEED = 100, 150, 2500000, (2025, 1)
+
+
+def case():
+    """The ligand, maps and job inputs of the fixture (shared with the test)."""
+    import oracle as O
+    from adgpu import synth
+    from adgpu._abi import CGrid, CLigand, energy_params, search_params
+    lig = synth.make_ligand(30, 6, seed=7)
+    maps = synth.make_maps(lig.type_names, npts=60, seed=11)
+    T = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+    packed = maps.packed()
+    ep = energy_params()
+    sp = search_params(num_runs=NRUN, pop_size=POP, max_evals=EVALS, ls="ad", spacing=maps.spacing)
+    init = O.initial_population(SEED, NRUN, POP, maps.size_angstr, maps.spacing)
+    seeds = O.cmwc_uints(SEED, NRUN * POP * 64)
+    cg = CGrid(packed, maps.size, maps.num_map_atypes, maps.spacing)
+    return dict(lig=lig, maps=maps, tables=T, packed=packed, ep=ep, sp=sp, init=init, seeds=seeds, cg=cg,
+                cl=CLigand(T))
+
+
+def main():
+    import oracle as O
+    threads = int(sys.argv[1]) if len(sys.argv) > 1 else (os.cpu_count() or 1)
+    c = case()
+    job = O.OracleJob(c["cg"], c["cl"], c["ep"], c["sp"], c["init"], c["seeds"])
+    t0 = time.time()
+    job.initpop()
+    gens, total = job.run(nthreads=threads)
+    st = job.state()
+    ene = st["ene"].reshape(NRUN, POP)
+    pop = st["pop"].reshape(NRUN, POP, 64)
+    bi = np.argmin(ene, axis=1)
+    best_e = ene[np.arange(NRUN), bi]
+    best_g = pop[np.arange(NRUN), bi]
+    np.savez_compressed(os.path.join(HERE, "lga_c1_oracle.npz"), best_energy=best_e, best_genotype=best_g,
+                        generations=gens, total_evals=total, evals_runs=st["evals_runs"],
+                        nrun=NRUN, pop=POP, evals=EVALS, seed=np.array(SEED))
+    print("lga_c1_oracle.npz: %d generations, %d evals, %.0f s on %d threads, median best %.4f" %
+          (gens, total, time.time() - t0, threads, float(np.median(best_e))))
+
+
+if __name__ == "__main__":
+    main()
