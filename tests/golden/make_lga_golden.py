@@ -1,14 +1,15 @@
-"""Writes tests/golden/lga_c1_oracle.npz: the oracle's whole LGA (the FP32
+"""Writes tests/golden/lga_c1_oracle.npz (ADADELTA) and
+lga_c1_<sw|fire|adam>_oracle.npz: the oracle's whole LGA (the FP32
 restatement of the reference's device control flow, oracle/) on the
-30-atom / 6-torsion ligand (seed 7) and 60^3 maps (seed 11), ADADELTA,
-nrun 100, pop 150, 2.5 M evaluations per run, from fixed initial population
-and LCG seeds: each run's best energy and genotype, the generation count and
-the eval totals.  The oracle needs minutes of CPU for this, so the GPU test
+30-atom / 6-torsion ligand (seed 7) and 60^3 maps (seed 11), nrun 100, pop
+150, 2.5 M evaluations per run, from fixed initial population and LCG seeds:
+each run's best energy and genotype, the generation count and the eval
+totals.  The oracle needs minutes of CPU for this, so the GPU test
 (tests/test_gpu_lga.py) compares the device's run of the same job with the
 stored result.  Deterministic: the oracle's runs are independent, so the
 thread count does not change the result.
 
-Usage: python tests/golden/make_lga_golden.py [threads]
+Usage: python tests/golden/make_lga_golden.py [ad|sw|fire|adam] [threads]
 """
 from __future__ import annotations
 
@@ -26,7 +27,11 @@ sys.path.insert(0, os.path.join(REPO, "oracle"))
 NRUN, POP, EVALS, SEED = 100, 150, 2500000, (2025, 1)
 
 
-def case():
+def fixture_name(ls):
+    return "lga_c1_oracle.npz" if ls == "ad" else "lga_c1_%s_oracle.npz" % ls
+
+
+def case(ls="ad"):
     """The ligand, maps and job inputs of the fixture (shared with the test)."""
     import oracle as O
     from adgpu import synth
@@ -36,7 +41,7 @@ def case():
     T = synth.ligand_tables(lig, maps.spacing, maps.type_names)
     packed = maps.packed()
     ep = energy_params()
-    sp = search_params(num_runs=NRUN, pop_size=POP, max_evals=EVALS, ls="ad", spacing=maps.spacing)
+    sp = search_params(num_runs=NRUN, pop_size=POP, max_evals=EVALS, ls=ls, spacing=maps.spacing)
     init = O.initial_population(SEED, NRUN, POP, maps.size_angstr, maps.spacing)
     seeds = O.cmwc_uints(SEED, NRUN * POP * 64)
     cg = CGrid(packed, maps.size, maps.num_map_atypes, maps.spacing)
@@ -46,8 +51,9 @@ def case():
 
 def main():
     import oracle as O
-    threads = int(sys.argv[1]) if len(sys.argv) > 1 else (os.cpu_count() or 1)
-    c = case()
+    ls = sys.argv[1] if len(sys.argv) > 1 else "ad"
+    threads = int(sys.argv[2]) if len(sys.argv) > 2 else (os.cpu_count() or 1)
+    c = case(ls)
     job = O.OracleJob(c["cg"], c["cl"], c["ep"], c["sp"], c["init"], c["seeds"])
     t0 = time.time()
     job.initpop()
@@ -58,11 +64,11 @@ def main():
     bi = np.argmin(ene, axis=1)
     best_e = ene[np.arange(NRUN), bi]
     best_g = pop[np.arange(NRUN), bi]
-    np.savez_compressed(os.path.join(HERE, "lga_c1_oracle.npz"), best_energy=best_e, best_genotype=best_g,
+    np.savez_compressed(os.path.join(HERE, fixture_name(ls)), best_energy=best_e, best_genotype=best_g,
                         generations=gens, total_evals=total, evals_runs=st["evals_runs"],
                         nrun=NRUN, pop=POP, evals=EVALS, seed=np.array(SEED))
-    print("lga_c1_oracle.npz: %d generations, %d evals, %.0f s on %d threads, median best %.4f" %
-          (gens, total, time.time() - t0, threads, float(np.median(best_e))))
+    print("%s: %d generations, %d evals, %.0f s on %d threads, median best %.4f" %
+          (fixture_name(ls), gens, total, time.time() - t0, threads, float(np.median(best_e))))
 
 
 if __name__ == "__main__":
