@@ -192,9 +192,12 @@ static inline R cube_interp(const Cube* cb, const R* w) {
            cb->c[4] * w[4] + cb->c[5] * w[5] + cb->c[6] * w[6] + cb->c[7] * w[7];
 }
 
+/* calcenergy.cl:293-301.  Written as "not inside" so that NaN coordinates
+ * (a FIRE step from a zero gradient divides by its norm) count as outside,
+ * as the device kernels do, instead of forming a map index from NaN. */
 static inline int out_of_grid(const adgpu_grid* g, R x, R y, R z) {
-    return (x < 0) || (y < 0) || (z < 0) || (x >= (R)(g->size_x - 1)) ||
-           (y >= (R)(g->size_y - 1)) || (z >= (R)(g->size_z - 1));
+    return !((x >= 0) && (y >= 0) && (z >= 0) && (x < (R)(g->size_x - 1)) &&
+             (y < (R)(g->size_y - 1)) && (z < (R)(g->size_z - 1)));
 }
 
 /* ------------------------------------------------------------------------- */

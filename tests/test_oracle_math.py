@@ -164,6 +164,14 @@ def test_inter_node_values_and_out_of_grid():
         g = np.zeros(64, np.float32)
         g[0:3] = p
         assert o.energy64(g)[0] == 16777216.0
+    # NaN coordinates (a FIRE step from a zero gradient) count as outside the
+    # maps, as on the device: the penalty, no map index formed from NaN
+    for p in [(np.nan, 5, 5), (5, np.nan, 5), (5, 5, np.nan)]:
+        g = np.zeros(64, np.float32)
+        g[0:3] = p
+        assert o.energy64(g)[0] == 16777216.0 and o.energy32(g)[0] == 16777216.0
+        e, grad = o.energrad32(g)
+        assert e == 16777216.0
 
 
 # --------------------------------------------------------------------------
