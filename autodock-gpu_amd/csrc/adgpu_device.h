@@ -982,7 +982,13 @@ struct Shape {
     int threads;   // workgroup size (64, 128 or 256)
 };
 
-constexpr size_t kLdsBudget = 64 * 1024;
+// Pair records are staged in LDS only while at least three workgroups still
+// fit in a compute unit's 160 KB: beyond that, reading them through L1/L2
+// keeps the four workgroups the registers allow.  Measured (M evals/s, FIRE /
+// ADADELTA): 60 atoms in 64-lane teams, 57 KB in LDS (2 workgroups) 145 / 139
+// against 172 / 167 from memory; 50 atoms in 32-lane teams, 49 KB in LDS (3
+// workgroups) 236 / 204 against 206 / 192 from memory.
+constexpr size_t kLdsBudget = 160 * 1024 / 3;
 
 // Chooses the team width and atom chunks for a ligand: the requested width,
 // else 32-lane teams (two individuals per wavefront) up to 128 atoms and
