@@ -640,8 +640,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded ligand and maps generated in-process)",
-        "config": {"workload": "LGA ADADELTA, 1 ligand of 30 atoms / 6 torsions, 60^3 maps at 0.375 A (%s), "
-                               "nrun %d, pop %d, %d evals per run" % (a.maps.upper(), a.nrun, a.pop, a.evals),
+        "config": {"workload": "LGA %s, 1 ligand of 30 atoms / 6 torsions, 60^3 maps at 0.375 A (%s), "
+                               "nrun %d, pop %d, %d evals per run"
+                               % ({"ad": "ADADELTA", "sw": "Solis-Wets", "adam": "ADAM", "fire": "FIRE"}.get(a.ls, a.ls),
+                                  a.maps.upper(), a.nrun, a.pop, a.evals),
                    "nrun": a.nrun, "pop_size": a.pop, "evals_per_run": a.evals, "ls": a.ls, "maps": a.maps,
                    "hw_queues": a.hw_queues},
         "ligands_per_hour": 3600.0 / float(np.mean(lat_arr)),
