@@ -98,7 +98,7 @@ struct adgpu_job {
     // serving slot: the ligand block belongs to the job (not a context id)
     bool own_lig = false;
     int want_groups = 1;                // run groups of a serving slot (fewer when a request has fewer runs)
-    bool shared = false;                // a serving slot: other requests share the device
+    int sharers = 1;                    // requests that may run at once with this job (a server's slots)
     int team_hint = 0;                  // team width of the job's kernels, fixed when it starts
     LigandSet own{};
     size_t own_cap = 0;
@@ -166,26 +166,30 @@ LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, 
     return la;
 }
 
-// Local searches of one generation below which a job on its own leaves most
-// of the device idle: 4096 searches in 32-lane teams are 2048 wavefronts,
-// half of the 4096 wavefront slots of 256 compute units at 4 waves per SIMD.
-constexpr int64_t kFillSearches = 4096;
+// Local searches in flight per generation below which 64-lane teams are
+// faster for ligands of 33-128 atoms.  With one team per wavefront a
+// launch's end does not leave half-idle wavefronts, which pays while the
+// searches are few enough for the launch tails to matter; with more, 32-lane
+// teams (two per wavefront) do the same work in fewer instructions.
+constexpr int64_t kWideTeamSearches = 20000;
 
 // A job's team width, fixed when it starts: the context's width or table,
-// else 32 lanes up to 32 atoms and 64 above 128; between, 64 lanes (one atom
-// chunk, twice the wavefronts) when the job's own searches leave the device
-// mostly idle and nothing else shares it, else 32 lanes (the same work in
-// fewer instructions on a full device).  Measured, seconds per ligand or
-// served ligands/h: lognormal 8-90-atom mix, FIRE nrun 20 (3000 searches),
-// alone 0.101 s (32) vs 0.081 s (64); served by 32 slots 85.4 k/h (32) vs
-// 82.8 k/h (64 from 33 atoms); 10-60 atoms, ADAM nrun 50 (7500 searches),
-// alone 0.207 s (32) vs 0.220 s (64).
-int job_team_width(adgpu_context* c, int num_atoms, int64_t searches, bool shared) {
+// else 32 lanes up to 32 atoms and 64 above 128; between, 64 lanes when the
+// searches in flight -- the job's own times the requests that may share the
+// device with it (a server's slots) -- are below kWideTeamSearches, else 32.
+// Measured, one job alone: lognormal 8-90-atom mix, FIRE nrun 20 (3000
+// searches) 0.101 s per ligand (32) vs 0.081 s (64); ligand + 2 flexible
+// residues (54 atoms), ADADELTA nrun 100 (15000 searches) 198 vs 226 M
+// evals/s; ADAM nrun 64 (9600 searches) at 50 / 60 atoms 240 / 164 vs 260 /
+// 176 M evals/s.  Shared: 4 slots x ADAM nrun 50 (30000 searches), 10-60
+// atoms, 16.0 vs 14.5 k ligands/h; 32 slots x FIRE nrun 20 (96000 searches),
+// 85.4 vs 82.8 k ligands/h (64 from 33 atoms).
+int job_team_width(adgpu_context* c, int num_atoms, int64_t searches, int sharers) {
     const int w = context_team_width(c, num_atoms);
     if (w) return w;
     if (num_atoms <= 32) return 32;
     if (num_atoms > 128) return 64;
-    return (!shared && searches < kFillSearches) ? 64 : 32;
+    return searches * std::max(1, sharers) < kWideTeamSearches ? 64 : 32;
 }
 
 // Message of the calling thread's last failed call (jobs of one context may
@@ -725,7 +729,7 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
     adgpu_context* c = j->ctx;
     j->sp = *sp;
     j->sd = to_dev(sp, lig.num_genes);
-    j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->shared);
+    j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
     for (auto& g : j->groups) {
@@ -1447,13 +1451,13 @@ size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params
     return b;
 }
 
-int slot_open(adgpu_context* c, int run_groups, bool shared, adgpu_job** out) {
+int slot_open(adgpu_context* c, int run_groups, int sharers, adgpu_job** out) {
     *out = nullptr;
     HIPCHK(c, hipSetDevice(c->device));
     std::unique_ptr<adgpu_job> j(new adgpu_job());
     j->ctx = c;
     j->own_lig = true;
-    j->shared = shared;
+    j->sharers = std::max(1, sharers);
     j->want_groups = std::max(1, std::min(run_groups, kMaxGroups));
     int rc = job_open(j.get());
     if (!rc) rc = job_set_groups(j.get(), run_groups, run_groups);

@@ -176,7 +176,7 @@ int adgpu_server_create(adgpu_ctx ctx, const adgpu_server_config* cfg, adgpu_ser
     s->t0 = Clock::now();
     for (int k = 0; k < cfg->slots; ++k) {
         adgpu_job* j = nullptr;
-        int rc = slot_open(ctx, cfg->run_groups > 0 ? cfg->run_groups : 1, cfg->slots > 1, &j);
+        int rc = slot_open(ctx, cfg->run_groups > 0 ? cfg->run_groups : 1, cfg->slots, &j);
         if (rc) {
             const std::string msg = thread_error();
             for (adgpu_job* o : s->slots) adgpu_job_destroy(o);

@@ -37,9 +37,9 @@ int map_hold(adgpu_context* c, int map_id, int delta);
 // scratch).
 size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params* sp, int as_frequency);
 
-// shared: other slots' requests run on the device at the same time (the
-// job's team width then assumes a full device).
-int slot_open(adgpu_context* c, int run_groups, bool shared, adgpu_job** out);
+// sharers: requests that may run on the device at the same time (the
+// server's slots); the job's team width counts their searches too.
+int slot_open(adgpu_context* c, int run_groups, int sharers, adgpu_job** out);
 // Binds the slot to a request: holds a use of the map set, grows buffers
 // that are too small, uploads the ligand block, the initial population and
 // the PRNG seeds, and resets the generation state.

@@ -164,9 +164,9 @@ const char* adgpu_last_error(adgpu_ctx ctx);
 const char* adgpu_version(void);
 /* Lanes that evaluate one individual: 0 = chosen per ligand (the team table
  * below if set; else for a job 32 up to 32 atoms, 64 above 128 atoms, and
- * between them 64 when the job's own local searches of a generation (fewer
- * than 4096) leave the device mostly idle and no other request shares it,
- * else 32; for per-pose calls 32 up to 128 atoms), 16 (four individuals per
+ * between them 64 while the local searches in flight per generation -- the
+ * job's own times a server's slots -- stay below 20000, else 32; for
+ * per-pose calls 32 up to 128 atoms), 16 (four individuals per
  * wavefront, ligands of at most 32 atoms; wider teams otherwise), 32 (two
  * individuals per wavefront) or 64.  Plays the role of the reference's
  * build-time NUM_OF_THREADS_PER_BLOCK (common/defines.h:29-49) as a run-time
