@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -127,7 +128,13 @@ void worker(adgpu_server* s, int slot) {
             r->res.slot = slot;
         }
         s->cv_work.notify_one();   // the next request may fit beside this one
-        run_request(s, j, r);
+        try {
+            run_request(s, j, r);
+        } catch (const std::exception& ex) {   // host memory for the results, for instance
+            slot_unbind(j);
+            r->res.status = ADGPU_ERR_NOMEM;
+            std::snprintf(r->res.message, sizeof(r->res.message), "%s", ex.what());
+        }
         (void)map_hold(s->ctx, r->map_id, -1);
         {
             std::lock_guard<std::mutex> lk(s->mu);
