@@ -68,6 +68,12 @@ class Context:
         most 32 atoms), 32 = two, 64 = one."""
         self._check(self.lib.adgpu_set_team_width(self.h, int(width)), "adgpu_set_team_width")
 
+    def set_ls_handoff(self, iterations: int):
+        """ADADELTA searches still running after this many iterations pass to
+        idle teams while unstarted searches remain (0: never).  Results are
+        bit-identical either way; applies to jobs created afterwards."""
+        self._check(self.lib.adgpu_set_ls_handoff(self.h, int(iterations)), "adgpu_set_ls_handoff")
+
     def set_team_table(self, table):
         """Per-size team widths [(max_atoms, width), ...] ascending; [] clears."""
         a = np.array([int(m) for m, _ in table] or [0], np.int32)

@@ -208,6 +208,7 @@ def load_library(path: str | None = None):
     lib.adgpu_close.argtypes = [vp]
     lib.adgpu_set_team_width.argtypes = [vp, C.c_int]
     lib.adgpu_set_team_table.argtypes = [vp, C.c_int, _ip, _ip]
+    lib.adgpu_set_ls_handoff.argtypes = [vp, C.c_int]
     lib.adgpu_maps_upload.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(C.c_int)]
     lib.adgpu_maps_release.argtypes = [vp, C.c_int]
     lib.adgpu_ligand_upload.argtypes = [vp, C.POINTER(Ligand), C.POINTER(EnergyParams), C.POINTER(C.c_int)]
@@ -246,6 +247,7 @@ def load_library(path: str | None = None):
 
 
 EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "adgpu_set_team_width",
+            "adgpu_set_ls_handoff",
             "adgpu_set_team_table", "adgpu_device_count", "adgpu_job_run_generations", "adgpu_rescore_f64", "adgpu_job_best_f64", "adgpu_maps_upload",
             "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses",
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",

@@ -50,6 +50,7 @@ struct adgpu_context {
     int next_id = 1;
     std::mutex mu;
     std::atomic<int> team_hint{0};   // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
+    std::atomic<int> ls_yield{32};   // ADADELTA hand-off point in iterations, 0 = off (ADGPU_LS_YIELD)
     std::mutex table_mu;
     std::vector<std::pair<int, int>> team_table;   // (max atoms, width), ascending; adgpu_set_team_table
     // scratch for adgpu_eval_poses
@@ -115,6 +116,9 @@ struct adgpu_job {
     int32_t* counters = nullptr;        // one LS work-queue counter per group (16 ints apart)
     int32_t* ls_ent = nullptr;
     float* ls_grad = nullptr;
+    float* ls_state = nullptr;          // ADADELTA hand-off: saved searches, ring (cap_yield LS slots)
+    int32_t* ls_ring = nullptr;
+    size_t cap_yield = 0;
     int32_t* h_ring = nullptr;          // pinned [kRing][num_runs] eval totals per generation
     std::vector<RunGroup> groups;
     int64_t generation = 0;
@@ -384,6 +388,13 @@ int adgpu_set_team_width(adgpu_ctx c, int width) {
     return ADGPU_OK;
 }
 
+int adgpu_set_ls_handoff(adgpu_ctx c, int iterations) {
+    if (!c) return ADGPU_ERR_ARG;
+    if (iterations < 0) return fail(c, ADGPU_ERR_ARG, "hand-off iterations must be >= 0");
+    c->ls_yield = iterations;
+    return ADGPU_OK;
+}
+
 int adgpu_set_team_table(adgpu_ctx c, int n, const int32_t* max_atoms, const int32_t* widths) {
     if (!c || n < 0 || n > 16 || (n > 0 && (!max_atoms || !widths)))
         return fail(c, ADGPU_ERR_ARG, "team table: n must be 0..16 with both arrays");
@@ -433,6 +444,7 @@ int adgpu_open(int device, adgpu_ctx* out) {
         }
         g_wait_event_flags = hipEventDisableTiming | (v == "block" ? hipEventBlockingSync : 0u);
     }
+    if (const char* y = std::getenv("ADGPU_LS_YIELD")) c->ls_yield = std::max(0, std::atoi(y));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
     if (const char* tw = std::getenv("ADGPU_TEAM_WIDTH")) {
         int v = std::atoi(tw);
@@ -674,7 +686,7 @@ static int job_open(adgpu_job* j) {
 // Grows the population, energy, PRNG and FIRE buffers to hold nrun runs of
 // pop individuals with nls local-search slots per run.  Buffers that already
 // fit are kept; growing frees the old ones (hipFree synchronizes the device).
-static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire) {
+static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire, bool yield) {
     const size_t ne = (size_t)nrun * pop;
     if (ne > j->cap_ne) {
         if (j->cap_ne) j->reallocs++;
@@ -718,7 +730,23 @@ static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire) {
         if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
         j->cap_ls = nls_all;
     }
+    if (yield && nls_all > j->cap_yield) {
+        if (j->cap_yield) j->reallocs++;
+        (void)hipFree(j->ls_state); (void)hipFree(j->ls_ring);
+        j->ls_state = nullptr;
+        j->ls_ring = nullptr;
+        j->cap_yield = 0;
+        bool ok = hipMalloc(&j->ls_state, nls_all * kAdStateFloats * sizeof(float)) == hipSuccess &&
+                  hipMalloc(&j->ls_ring, nls_all * sizeof(int32_t)) == hipSuccess;
+        if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
+        j->cap_yield = nls_all;
+    }
     return ADGPU_OK;
+}
+
+// True when the job's ADADELTA searches hand off between teams.
+static bool yield_wanted(const adgpu_context* c, const adgpu_search_params* sp) {
+    return sp->ls_method == ADGPU_LS_AD && c->ls_yield.load() > 0;
 }
 
 // Starts a search: uploads the initial population and PRNG seeds, clears
@@ -729,6 +757,8 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
     adgpu_context* c = j->ctx;
     j->sp = *sp;
     j->sd = to_dev(sp, lig.num_genes);
+    if (yield_wanted(c, sp) && j->cap_yield >= (size_t)sp->num_runs * std::max(1, j->sd.num_ls))
+        j->sd.ls_yield = c->ls_yield.load();
     j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
@@ -797,7 +827,9 @@ int adgpu_job_create(adgpu_ctx c, int map_id, int lig_id, const adgpu_search_par
     if (const char* e = std::getenv("ADGPU_RUN_GROUPS")) ng = std::atoi(e);
     rc = job_open(j.get());
     if (!rc) rc = job_set_groups(j.get(), ng, nrun);
-    if (!rc) rc = job_reserve(j.get(), nrun, sp->pop_size, to_dev(sp, num_genes).num_ls, sp->ls_method == ADGPU_LS_FIRE);
+    if (!rc)
+        rc = job_reserve(j.get(), nrun, sp->pop_size, to_dev(sp, num_genes).num_ls, sp->ls_method == ADGPU_LS_FIRE,
+                         yield_wanted(c, sp));
     if (!rc) rc = job_start(j.get(), sp, li->second.host, init_pop, seeds);
     if (rc) {
         const std::string msg = t_err;
@@ -826,6 +858,7 @@ int adgpu_job_destroy(adgpu_job_t j) {
     (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
     (void)hipFree(j->evals_new); (void)hipFree(j->evals_runs); (void)hipFree(j->prng);
     (void)hipFree(j->counters); (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
+    (void)hipFree(j->ls_state); (void)hipFree(j->ls_ring);
     (void)hipFree(j->d_stop); (void)hipFree(j->d_as); (void)hipFree(j->d_as_bins);
     (void)hipFree(j->d_as_log); (void)hipFree(j->d_as_out);
     (void)hipFree(j->own.dev); (void)hipFree(j->d_best); (void)hipFree(j->d_f64);
@@ -871,6 +904,8 @@ static JobBuffers slice(const adgpu_job* j, int r0, int r1, int parity, int coun
     const size_t nls = (size_t)(j->sd.num_ls > 0 ? j->sd.num_ls : 1);
     b.ls_ent = j->ls_ent ? j->ls_ent + (size_t)r0 * nls : nullptr;
     b.ls_grad = j->ls_grad ? j->ls_grad + (size_t)r0 * nls * kGenes : nullptr;
+    b.ls_state = j->ls_state ? j->ls_state + (size_t)r0 * nls * kAdStateFloats : nullptr;
+    b.ls_ring = j->ls_ring ? j->ls_ring + (size_t)r0 * nls : nullptr;
     b.stop = j->d_stop;
     (void)r1;
     return b;
@@ -925,6 +960,18 @@ int adgpu_job_kernel_gen(adgpu_job_t j) {
     return ADGPU_OK;
 }
 
+// The ADADELTA hand-off's fault flags (one per group): a team that polled a
+// reserved ring entry past its bound stopped and left that search
+// unfinished.  Read once the job's kernels have completed.
+static int check_handoff(adgpu_job_t j) {
+    if (j->sd.ls_yield <= 0) return ADGPU_OK;
+    int32_t q[16 * kMaxGroups];
+    HIPCHK(j->ctx, hipMemcpy(q, j->counters, sizeof(q), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kMaxGroups; ++k)
+        if (q[16 * k + 3]) return fail(j->ctx, ADGPU_ERR_HIP, "ADADELTA hand-off between teams did not complete");
+    return ADGPU_OK;
+}
+
 int adgpu_job_kernel_ls(adgpu_job_t j, int generation) {
     if (!j) return ADGPU_ERR_ARG;
     adgpu_context* c = j->ctx;
@@ -938,7 +985,7 @@ int adgpu_job_kernel_ls(adgpu_job_t j, int generation) {
     HIPCHK(c, launch_ls(launch_args(c, *ls, *ms, s, 1, j->team_hint), j->sd,
                         slice(j, 0, j->sp.num_runs, whole_parity(j), 0), m));
     HIPCHK(c, hipStreamSynchronize(s));
-    return ADGPU_OK;
+    return check_handoff(j);
 }
 
 int adgpu_job_kernel_sum_and_swap(adgpu_job_t j) {
@@ -1172,6 +1219,7 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
         }
         j->stopped = true;
     }
+    if ((rc = check_handoff(j))) return rc;
     int64_t total = 0;
     for (auto& g : j->groups) {
         if (!stop_known && g.done != gens) return fail(c, ADGPU_ERR_STATE, "run groups stopped at different generations");
@@ -1442,6 +1490,10 @@ size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params
         const size_t nls = (size_t)std::max(1, to_dev(sp, lig.hd.num_genes).num_ls);
         b += nrun * nls * (sizeof(int32_t) + kGenes * sizeof(float));
     }
+    if (sp->ls_method == ADGPU_LS_AD) {   // saved searches and hand-off ring (when hand-off is on)
+        const size_t nls = (size_t)std::max(1, to_dev(sp, lig.hd.num_genes).num_ls);
+        b += nrun * nls * (kAdStateFloats * sizeof(float) + sizeof(int32_t));
+    }
     if (as_frequency > 0) {   // log_cap as adgpu_job_set_autostop sizes it
         const int64_t tests = sp->max_generations / as_frequency + 2;
         const size_t cap = (size_t)std::max<int64_t>(16, std::min<int64_t>(tests, 16384));
@@ -1486,7 +1538,7 @@ int slot_bind(adgpu_job* j, int map_id, const PreparedLigand& lig, const adgpu_s
     const int nrun = sp->num_runs;
     const SearchDev sd = to_dev(sp, lig.hd.num_genes);
     int rc = job_set_groups(j, j->want_groups, nrun);
-    if (!rc) rc = job_reserve(j, nrun, sp->pop_size, sd.num_ls, sp->ls_method == ADGPU_LS_FIRE);
+    if (!rc) rc = job_reserve(j, nrun, sp->pop_size, sd.num_ls, sp->ls_method == ADGPU_LS_FIRE, yield_wanted(c, sp));
     if (!rc && lig.blob.size() > j->own_cap) {
         if (j->own_cap) j->reallocs++;
         (void)hipFree(j->own.dev);

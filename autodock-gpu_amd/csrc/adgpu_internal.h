@@ -64,8 +64,15 @@ struct SearchDev {
     float abs_max_dmov, abs_max_dang, lsearch_rate;        // lsearch_rate in percent
     float rho_lower_bound, base_dmov_mul_sqrt3, base_dang_mul_sqrt3;
     int32_t cons_limit, max_iters, ls_method;
-    float adam_beta1, adam_beta2, adam_epsilon, pad;
+    float adam_beta1, adam_beta2, adam_epsilon;
+    int32_t ls_yield;   // ADADELTA: iterations after which a search may hand off (0: never)
 };
+
+// State of an ADADELTA search handed off between teams: genes, best genes,
+// E[g^2], E[dx^2] (kGenes floats each), then best energy, varrho,
+// consecutive successes, consecutive failures, iterations and the
+// population index (integers stored as their bit patterns).
+constexpr int kAdStateFloats = 4 * kGenes + 8;
 
 struct JobBuffers {
     float* pop_cur;
@@ -75,9 +82,13 @@ struct JobBuffers {
     int32_t* evals_new;
     int32_t* evals_runs;
     uint32_t* prng;
-    int32_t* ls_counter;   // work queue of the persistent local-search kernels
+    int32_t* ls_counter;   // work queue of the persistent local-search kernels: [0] next unstarted
+                           // search; ADADELTA hand-off: [2] next ring ticket, [3] fault flag
+                           // (sticky over a run)
     int32_t* ls_ent;       // FIRE: entity chosen per LS slot [runs*num_ls]
     float* ls_grad;        // FIRE: initial gradient per LS slot [runs*num_ls*64]
+    float* ls_state;       // ADADELTA hand-off: saved search per LS slot [runs*num_ls*kAdStateFloats]
+    int32_t* ls_ring;      // ADADELTA hand-off: per claim of the queue, the search handed off there [runs*num_ls]
     const int32_t* stop;   // set by a passed AutoStop test: generation kernels return at once
 };
 

@@ -206,4 +206,52 @@ DEV void ad_store(const AdItem<TW>& s, int NG, float* genes, float* ene, int32_t
     if (tl == 0) { *evals += s.iters; *ene = s.best_e; }
 }
 
+// Saves a search in progress and its population index to rec
+// (kAdStateFloats floats); ad_load restores both exactly, so a search
+// continued by another team computes what its first team would have.  The
+// record moves between compute units of different XCDs within one launch:
+// it is written and read with device-scope accesses (sc1: coherent across
+// the XCDs' L2 caches without writing back or invalidating them).
+DEV void st_dev(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DEV float ld_dev(const float* p) {
+    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int TW>
+DEV void ad_save(const AdItem<TW>& s, int gi, float* rec) {
+    const int tl = Team<TW>::lane();
+#pragma unroll
+    for (int j = 0; j < 64 / TW; ++j) {
+        const int g = tl + TW * j;
+        st_dev(rec + g, s.geno[j]);
+        st_dev(rec + kGenes + g, s.best[j]);
+        st_dev(rec + 2 * kGenes + g, s.sqg[j]);
+        st_dev(rec + 3 * kGenes + g, s.sqd[j]);
+    }
+    if (tl == 0) {
+        float* r = rec + 4 * kGenes;
+        st_dev(r + 0, s.best_e); st_dev(r + 1, s.varrho);
+        st_dev(r + 2, __int_as_float(s.cons_succ)); st_dev(r + 3, __int_as_float(s.cons_fail));
+        st_dev(r + 4, __int_as_float(s.iters)); st_dev(r + 5, __int_as_float(gi));
+    }
+}
+
+template <int TW>
+DEV int ad_load(AdItem<TW>& s, const float* rec) {
+    const int tl = Team<TW>::lane();
+#pragma unroll
+    for (int j = 0; j < 64 / TW; ++j) {
+        const int g = tl + TW * j;
+        s.geno[j] = ld_dev(rec + g);
+        s.best[j] = ld_dev(rec + kGenes + g);
+        s.sqg[j] = ld_dev(rec + 2 * kGenes + g);
+        s.sqd[j] = ld_dev(rec + 3 * kGenes + g);
+    }
+    const float* r = rec + 4 * kGenes;
+    s.best_e = ld_dev(r + 0); s.varrho = ld_dev(r + 1);
+    s.cons_succ = __float_as_int(ld_dev(r + 2)); s.cons_fail = __float_as_int(ld_dev(r + 3));
+    s.iters = __float_as_int(ld_dev(r + 4));
+    return __float_as_int(ld_dev(r + 5));
+}
+
 }  // namespace adgpu

@@ -59,10 +59,14 @@ __global__ __launch_bounds__(64) void k_sum_evals(SearchDev S, JobBuffers B) {
     if (lane == 0) B.evals_runs[run] += s;
 }
 
-// Clears the local-search work queue (a kernel rather than a memset, so the
-// launch sequence can be captured into a graph).
-__global__ __launch_bounds__(64) void k_reset_queue(int32_t* counter) {
-    if (threadIdx.x == 0) *counter = 0;
+// Clears the local-search work queue and, for ADADELTA hand-offs, its
+// counters and the n entries of the hand-off ring (a kernel rather than a
+// memset, so the launch sequence can be captured into a graph).  The fault
+// flag counter[3] stays: it is cleared when a search starts.
+__global__ __launch_bounds__(256) void k_reset_queue(int32_t* counter, int32_t* ring, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3) counter[i] = 0;
+    if (ring != nullptr && i < n) ring[i] = -1;
 }
 
 hipError_t launch_initpop(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b) {
@@ -104,7 +108,10 @@ hipError_t launch_sum_evals(hipStream_t s, const SearchDev& sp, const JobBuffers
 
 hipError_t launch_ls(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b, int method) {
     if (sp.num_runs * sp.num_ls <= 0 || method == ADGPU_LS_NONE) return hipSuccess;
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, la.stream, b.ls_counter);
+    const int n = sp.num_runs * sp.num_ls;
+    int32_t* ring = method == ADGPU_LS_AD && sp.ls_yield > 0 ? b.ls_ring : nullptr;
+    hipLaunchKernelGGL(k_reset_queue, dim3(ring != nullptr ? (n + 255) / 256 : 1), dim3(256), 0, la.stream,
+                       b.ls_counter, ring, n);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     switch (method) {

@@ -8,8 +8,69 @@
 
 namespace adgpu {
 
-// Each team takes LS slots (run, entity) from the queue until it is empty
-// and runs one ADADELTA search per slot (adgpu_items.h).
+// Polls a team makes for one ring entry before it reports a fault and stops
+// (an entry is written right after its claim, and claims all precede the
+// first ticket, so this bound is never reached by a correct run).
+constexpr int kHandoffPolls = 1 << 22;
+
+// s_waitcnt vmcnt(0): every vector memory access of the wavefront, the
+// saved record's device-scope stores among them, has completed.  The hand-off
+// is ordered this way rather than by release/acquire fences, which write back
+// and invalidate the whole L2 of an XCD (and with it the grid maps every
+// evaluation reads).
+constexpr int kWaitStores = 0x0F70;
+
+// Ring entries: one per LS slot claimed from the queue, written by the team
+// that claimed it, right after the claim.
+constexpr int kRingUnset = -1;   // not written yet (cleared before the launch)
+constexpr int kRingNone = -2;    // claimed without a hand-off
+
+// Team lane 0: takes tickets over the ring in claim order.  Returns the LS
+// slot of a handed-off search to resume, -1 when the team should look again in
+// its next iteration (its ticket's entry is not written yet), -2 when the
+// tickets are used up, -3 on a fault.  `ticket` keeps lane 0's ticket across
+// calls (-1: none).
+DEV int take_handoff(int32_t* Q, const int32_t* ring, int ntot, int& ticket, int& polls) {
+    for (int k = 0; k < 4; ++k) {   // a few entries without a hand-off per call
+        if (ticket < 0) {
+            ticket = atomicAdd(&Q[2], 1);
+            polls = 0;
+        }
+        if (ticket >= ntot) return -2;
+        const int v = __hip_atomic_load(const_cast<int32_t*>(&ring[ticket]), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        if (v >= 0 && v < ntot) {
+            ticket = -1;
+            return v;
+        }
+        if (v == kRingUnset) {
+            if (++polls > kHandoffPolls) {
+                __hip_atomic_store(&Q[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return -3;
+            }
+            return -1;
+        }
+        ticket = -1;   // kRingNone: no hand-off at this claim
+    }
+    return -1;
+}
+
+// Each team takes LS slots (run, entity) from the queue and runs one
+// ADADELTA search per slot (adgpu_items.h).
+//
+// Hand-off (S.ls_yield > 0 and a ring): searches last from one to max_iters
+// iterations, so teams that drew long searches late would finish the launch
+// alone.  A search that reaches ls_yield iterations while unstarted slots
+// remain is saved to ls_state, and its team starts the unstarted slot it
+// claimed.  The ring has one entry per claim: the claiming team writes the
+// handed-off search there, or kRingNone.  Once no unstarted slot is left,
+// teams take tickets over the ring in claim order (one atomic add each, no
+// compare-and-swap for many teams to contend on) and resume the searches they
+// find.  Each search is handed off at most once, and a resumed search
+// computes exactly what it would have computed without the hand-off.  Every
+// claim precedes the first ticket, so a ticket waits only for an entry its
+// claimer is about to write, and waits by looking once per iteration, so its
+// wavefront partner keeps running.
 template <int TW, int NC, int MF, bool PL>
 __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
     if (JOB_STOPPED(a.B)) return;
@@ -17,35 +78,98 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
     const SearchDev& S = a.S;
     const int nls = S.num_ls, pop = S.pop_size, NG = S.num_genes;
     const int ntot = S.num_runs * nls;
+    int32_t* const Q = a.B.ls_counter;
+    int32_t* const ring = a.B.ls_ring;
+    const int yq = ring != nullptr && a.B.ls_state != nullptr ? S.ls_yield : 0;
+    // While a team has no search, its AdItem is dead: cons_succ and cons_fail
+    // then hold lane 0's ring ticket (-1: none) and its poll count.
     AdItem<TW> it;
 #pragma unroll
     for (int j = 0; j < GPL; ++j) { it.geno[j] = 0.f; it.best[j] = 0.f; it.sqg[j] = 0.f; it.sqd[j] = 0.f; }
-    it.best_e = INFINITY; it.varrho = 1.0f; it.cons_succ = 0; it.cons_fail = 0; it.iters = 0;
-    size_t gi = 0;
-    bool done = false, need = true;
+    it.best_e = INFINITY; it.varrho = 1.0f; it.cons_succ = -1; it.cons_fail = 0; it.iters = 0;
+    int gi = 0;                      // population index of the search (runs * pop_size < 2^31)
+    int t = 0;                       // LS slot of the search in progress
+    bool active = false, done = false, need = true;
+    bool fresh = true;               // unstarted slots may remain
+    bool may_yield = false;          // the search in progress may still be handed off
     for (;;) {
         if (need) {   // uniform within the team
-            int t = 0;
-            if (tl == 0) t = atomicAdd(a.B.ls_counter, 1);
-            t = TM::bcast_i(t, 0);
-            need = false;
-            if (t >= ntot) {
+            int c = -1;
+            if (fresh) {
+                if (tl == 0) {
+                    c = atomicAdd(&Q[0], 1);
+                    if (yq > 0 && c < ntot) __hip_atomic_store(&ring[c], kRingNone, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
+                }
+                c = TM::bcast_i(c, 0);
+                if (c >= ntot) { fresh = false; c = -1; }
+            }
+            if (c < 0 && yq > 0) {
+                int v = -2;
+                if (tl == 0) v = take_handoff(Q, ring, ntot, it.cons_succ, it.cons_fail);
+                v = TM::bcast_i(v, 0);
+                if (v >= 0) {
+                    t = v;   // the record's loads depend on v: they cannot pass the ring load
+                    gi = ad_load<TW>(it, a.B.ls_state + (size_t)t * kAdStateFloats);
+                    may_yield = false;
+                    active = true;
+                    need = false;
+                } else {
+                    active = false;
+                    done = v != -1;
+                    need = !done;
+                }
+            } else if (c >= 0) {
+                t = c;
+                const int run = t / nls, le = t - run * nls;
+                gi = run * pop + ls_entity<TW>(S, le, a.B.prng + (size_t)t * 64);
+                ad_begin<TW>(it, a.B.pop_next + (size_t)gi * kGenes, NG);
+                may_yield = yq > 0;
+                active = true;
+                need = false;
+            } else {
+                active = false;
                 done = true;
+                need = false;
+            }
+            if (!active) {
 #pragma unroll
                 for (int j = 0; j < GPL; ++j) it.geno[j] = 0.f;
-            } else {
-                const int run = t / nls, le = t - run * nls;
-                const int ent = ls_entity<TW>(S, le, a.B.prng + (size_t)t * 64);
-                gi = (size_t)run * pop + ent;
-                ad_begin<TW>(it, a.B.pop_next + gi * kGenes, NG);
             }
         }
         if (!TM::any_in_wave(!done)) break;
+        if (!TM::any_in_wave(active)) {   // every team of the wavefront waits for a ring entry
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
         float g[GPL];
         const float e = evaluate<TW, NC, MF, true>(X, it.geno, g);
-        if (!done && ad_step<TW>(it, e, g, NG, S.max_iters)) {
-            ad_store<TW>(it, NG, a.B.pop_next + gi * kGenes, a.B.ene_next + gi, a.B.evals_new + gi);
+        if (!active) continue;
+        if (ad_step<TW>(it, e, g, NG, S.max_iters)) {
+            ad_store<TW>(it, NG, a.B.pop_next + (size_t)gi * kGenes, a.B.ene_next + gi, a.B.evals_new + gi);
+            it.cons_succ = -1;
+            it.cons_fail = 0;
+            active = false;
             need = true;
+        } else if (may_yield && it.iters == yq) {
+            may_yield = false;
+            int c = -1;
+            if (fresh) {
+                if (tl == 0) c = atomicAdd(&Q[0], 1);
+                c = TM::bcast_i(c, 0);
+                if (c >= ntot) { fresh = false; c = -1; }
+            }
+            if (c >= 0) {
+                // hand the search off at claim c (one entry per claim, so at most once per slot)
+                ad_save<TW>(it, gi, a.B.ls_state + (size_t)t * kAdStateFloats);
+                __builtin_amdgcn_s_waitcnt(kWaitStores);   // the record is written before the ring entry
+                if (tl == 0) __hip_atomic_store(&ring[c], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                t = c;
+                const int run = t / nls, le = t - run * nls;
+                gi = run * pop + ls_entity<TW>(S, le, a.B.prng + (size_t)t * 64);
+                ad_begin<TW>(it, a.B.pop_next + (size_t)gi * kGenes, NG);
+                may_yield = true;
+            }
         }
     }
 }

@@ -180,6 +180,16 @@ int  adgpu_set_team_width(adgpu_ctx ctx, int width);
  * adgpu.autotune fills it from measured local-search launches.  Set it before
  * creating the jobs that should use it. */
 int  adgpu_set_team_table(adgpu_ctx ctx, int n, const int32_t* max_atoms, const int32_t* widths);
+/* ADADELTA hand-off: a search that has run `iterations` iterations while
+ * unstarted searches of the same launch remain is saved and resumed later by
+ * whichever team is free, so long searches drawn late do not end the launch
+ * alone.  Every search computes exactly what it computes without hand-off
+ * (bit-identical results); 0 turns it off.  Default 32 (environment variable
+ * ADGPU_LS_YIELD at adgpu_open); applies to jobs created afterwards.  The
+ * reference runs each ADADELTA search to its end in a block of its own
+ * (cuda/kernel_ad.cu:60-120: one block per run and LS entity); this has no
+ * counterpart there and changes only when each search runs. */
+int  adgpu_set_ls_handoff(adgpu_ctx ctx, int iterations);
 
 /* --- resident inputs ---------------------------------------------------------
  * Replace the per-job uploads in docking_with_gpu

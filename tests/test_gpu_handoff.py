@@ -1,0 +1,85 @@
+"""ADADELTA hand-off between teams (adgpu_set_ls_handoff) through the C ABI.
+
+A search still running after the hand-off point, while unstarted searches of
+the launch remain, is saved to device memory and resumed later by whichever
+team is free.  Every search computes what it computes without hand-off, so a
+docking is bit-identical with hand-off off, after one iteration (every search
+that can be handed off is), or at the default 32: generation count, eval
+total, final population, energies, per-run eval counters and LCG streams.
+Ligand sizes and team widths cover 16-, 32- and 64-lane teams and one, two
+and four atom chunks per lane, where the kernel variants (and their register
+budgets) differ.  The reference runs each search to its end in its own block
+(cuda/kernel_ad.cu:60-120); its results do not depend on the order in which
+searches run, and neither do these.
+"""
+import numpy as np
+import pytest
+
+import adgpu
+from adgpu import synth
+from adgpu._abi import energy_params, search_params
+
+pytestmark = pytest.mark.gpu
+
+NRUN, POP, EVALS = 40, 150, 300000
+
+
+def _ligand(n_atoms, n_tors, seed):
+    for t in range(n_tors, -1, -1):
+        try:
+            return synth.make_ligand(n_atoms, t, seed=seed)
+        except RuntimeError:
+            continue
+    raise RuntimeError("no ligand")
+
+
+def _initial(nrun, pop, maps, seed):
+    rng = np.random.default_rng(seed)
+    n = nrun * pop
+    init = np.zeros((n, 64), np.float32)
+    init[:, 0:3] = rng.random((n, 3)) * (np.array(maps.size, np.float64) - 1.0)
+    init[:, 3:6] = rng.random((n, 3)) * np.array([360.0, 180.0, 360.0])
+    init[:, 6:] = rng.random((n, 58)) * 360.0
+    seeds = rng.integers(0, 2 ** 32, size=n * 64, dtype=np.uint64).astype(np.uint32)
+    return init, seeds
+
+
+def _dock(ctx, mid, lid, sp, init, seeds, handoff):
+    ctx.set_ls_handoff(handoff)
+    job = ctx.job(mid, lid, sp, init, seeds)
+    job.initpop()
+    gens, total = job.run()
+    st = job.download()
+    job.destroy()
+    return gens, total, st
+
+
+@pytest.mark.parametrize("atoms,tors,width", [(30, 6, 0), (20, 4, 16), (20, 4, 64), (70, 12, 32), (110, 20, 0),
+                                              (110, 20, 64)])
+def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
+    lig = _ligand(atoms, tors, 100 + atoms)
+    maps = synth.make_maps(lig.type_names, npts=48, spacing=0.375, seed=atoms)
+    tables = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+    mid = gpu_ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing)
+    lid = gpu_ctx.ligand_upload(tables, energy_params())
+    sp = search_params(num_runs=NRUN, pop_size=POP, max_evals=EVALS, ls="ad", spacing=maps.spacing)
+    init, seeds = _initial(NRUN, POP, maps, atoms)
+    gpu_ctx.set_team_width(width)
+    try:
+        ref = _dock(gpu_ctx, mid, lid, sp, init, seeds, 0)
+        for h in (1, 32):
+            print("atoms", atoms, "width", width, "hand-off", h, flush=True)
+            got = _dock(gpu_ctx, mid, lid, sp, init, seeds, h)
+            assert got[0] == ref[0] and got[1] == ref[1], (h, got[:2], ref[:2])
+            for k in ref[2]:
+                assert np.array_equal(got[2][k], ref[2][k]), (h, k)
+    finally:
+        gpu_ctx.set_ls_handoff(32)
+        gpu_ctx.set_team_width(0)
+        gpu_ctx.ligand_release(lid)
+        gpu_ctx.maps_release(mid)
+
+
+def test_handoff_argument(gpu_ctx):
+    with pytest.raises(adgpu.AdgpuError):
+        gpu_ctx.set_ls_handoff(-1)

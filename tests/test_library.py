@@ -46,6 +46,7 @@ def test_null_arguments_are_rejected_without_gpu():
     assert lib.adgpu_server_stats_get(None, None) == 1
     assert lib.adgpu_server_destroy(None) == 1
     assert lib.adgpu_set_team_table(None, 0, None, None) == 1
+    assert lib.adgpu_set_ls_handoff(None, 32) == 1
 
 
 def test_oracle_symbols():
