@@ -19,13 +19,21 @@
  *                  and smoothing plateau, invariance under rigid motion, and
  *                  the reference's table-based calc_intraE_f (oracle/_ref);
  *   energrad    -- central finite differences of the FP64 energy for the
- *                  translation and torsion genes; orientation genes follow the
- *                  empirical torque mapping of calcMergedEneGra.cl:712-861 and
- *                  are pinned by FP32-vs-FP64 agreement and by special cases
- *                  (zero torque -> zero gradient, rotation-only symmetry);
+ *                  translation and torsion genes; orientation genes (the
+ *                  torque mapping of calcMergedEneGra.cl:712-861, both
+ *                  branches) against their definition -- the genotype-angle
+ *                  change of a turn about the torque axis, computed with
+ *                  rotation matrices -- and fast_acos against acos;
+ *   gradient    -- (FIRE's float gradient) central differences of the
+ *                  analytic AD4 pair energies (Mehler-Solmajer, Gaussian
+ *                  desolvation, smoothed vdW, CG-G0) and of the FP64 energy
+ *                  where both use one function; orientation genes as above;
  *   GA / LS     -- injected LCG streams with hand-derived expectations
  *                  (selection, crossover points, mutation bounds, rho schedule,
- *                  update equations on a quadratic bowl).
+ *                  ADADELTA first step); ADAM and FIRE by closed-form
+ *                  trajectories on linear, V-shaped and kinked fields.
+ * tests/test_oracle_mutants.py plants a mistake in each of these functions
+ * (a copy of this file) and checks that a pin fails.
  */
 #include "adock_oracle.h"
 #include <stdlib.h>
@@ -104,6 +112,9 @@ static inline R fast_acos(R cosine) {
            e * sqrt((R)2.0f - sqrt((R)2.0f + (R)2.0f * x)) - f * sqrt((R)2.0f - (R)2.0f * x);
     return copysign(ac, cosine) + (cosine < (R)0.0f ? PI_F : (R)0.0f);
 }
+
+/* fast_acos on its own, so tests can bound it against acos. */
+R SFX(orc_fast_acos)(R cosine) { return fast_acos(cosine); }
 
 /* float2int_round, calcMergedEneGra.cl:30-35, applied to the clamped term
  * fmin(MAXTERM, fmax(-MAXTERM, x)) as every call site does.  Note the

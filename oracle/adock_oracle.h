@@ -58,6 +58,11 @@ void   orc_gradient_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_
 void   orc_coords_f32(const adgpu_ligand* L, const float* genotype, float* xyz);
 void   orc_coords_f64(const adgpu_ligand* L, const float* genotype, double* xyz);
 
+/* fast_acos (calcenergy.cl:93-108), the polynomial arc cosine of the
+ * orientation gradient. */
+float  orc_fast_acos_f32(float cosine);
+double orc_fast_acos_f64(double cosine);
+
 /* ---- FP32-only: device control flow ------------------------------------ */
 
 /* One LCG draw from state *s (auxiliary_genetic.cl:29-67). */

@@ -45,7 +45,9 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
-        p = os.path.join(HERE, "liboracle.so")
+        # ADGPU_ORACLE_LIB names another build of the oracle (the mutation
+        # tests load deliberately broken copies through it)
+        p = os.environ.get("ADGPU_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(p):
             raise RuntimeError("oracle not built: make -C oracle")
         L = C.CDLL(p)
@@ -62,6 +64,8 @@ def lib():
         L.orc_atom_grad_f64.argtypes = [gp, lp, ep, _fp, _dp, _dp]
         L.orc_gradient_f32.argtypes = [gp, lp, ep, _fp, _fp]
         L.orc_gradient_f64.argtypes = [gp, lp, ep, _fp, _dp]
+        L.orc_fast_acos_f64.restype = C.c_double
+        L.orc_fast_acos_f64.argtypes = [C.c_double]
         L.orc_coords_f32.argtypes = [lp, _fp, _fp]
         L.orc_coords_f64.argtypes = [lp, _fp, _dp]
         L.orc_rand.restype = C.c_uint32
@@ -135,6 +139,13 @@ class Oracle:
         s = C.c_double()
         lib().orc_atom_grad_f64(*self._args(), g.ctypes.data_as(_fp), None, C.byref(s))
         return float(s.value)
+
+    def atom_grad64(self, geno):
+        """Per-atom dE/dx (kcal/mol/A, unquantized), shape (atoms, 3)."""
+        g = _f(geno)
+        out = np.zeros(3 * self.n, np.float64)
+        lib().orc_atom_grad_f64(*self._args(), g.ctypes.data_as(_fp), out.ctypes.data_as(_dp), None)
+        return out.reshape(-1, 3)
 
     def gradient32(self, geno):
         g = _f(geno)
