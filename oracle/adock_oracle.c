@@ -645,6 +645,55 @@ void SFX(orc_gradient)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_e
     for (int k = 0; k < NG; ++k) grad[k] = (k > 2) ? gg[k] * conv : gg[k];
 }
 
+/* ------------------------------------------------------------------------- */
+/* Batches of genotypes (stride 64), split over nthreads OpenMP threads: the  */
+/* per-pose functions above, one call per pose, for parity runs over 1e5      */
+/* poses.  Optional outputs may be NULL.                                      */
+/* ------------------------------------------------------------------------- */
+void SFX(orc_energy_batch)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                           const float* genotypes, int64_t n, R* energies, R* inters, R* abs_sums,
+                           int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        R a = 0, it = 0;
+        R e = SFX(orc_energy)(g, L, ep, genotypes + i * ADGPU_GENOTYPE_STRIDE, &it, NULL, &a);
+        energies[i] = e;
+        if (inters) inters[i] = it;
+        if (abs_sums) abs_sums[i] = a;
+    }
+}
+
+void SFX(orc_energrad_batch)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                             const float* genotypes, int64_t n, int quantize, R* energies, R* grads,
+                             int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        R gr[ADGPU_GENOTYPE_STRIDE];
+        memset(gr, 0, sizeof(gr));
+        R e = SFX(orc_energrad)(g, L, ep, genotypes + i * ADGPU_GENOTYPE_STRIDE, quantize, gr);
+        if (energies) energies[i] = e;
+        if (grads) memcpy(grads + i * ADGPU_GENOTYPE_STRIDE, gr, sizeof(gr));
+    }
+}
+
+void SFX(orc_gradient_batch)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                             const float* genotypes, int64_t n, R* grads, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        R gr[ADGPU_GENOTYPE_STRIDE];
+        memset(gr, 0, sizeof(gr));
+        SFX(orc_gradient)(g, L, ep, genotypes + i * ADGPU_GENOTYPE_STRIDE, gr);
+        memcpy(grads + i * ADGPU_GENOTYPE_STRIDE, gr, sizeof(gr));
+    }
+}
+
+void SFX(orc_sensitivity_batch)(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                                const float* genotypes, int64_t n, R* sensitivity, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i)
+        SFX(orc_atom_grad)(g, L, ep, genotypes + i * ADGPU_GENOTYPE_STRIDE, NULL, &sensitivity[i]);
+}
+
 #if !defined(ORC_DOUBLE)
 /* ========================================================================= */
 /* FP32-only: GA, local searches, LGA driver                                  */
@@ -718,20 +767,20 @@ void orc_sum_evals(orc_job* j, int r0, int r1) {
 
 /* gpu_gen_and_eval_newpops (kernel4.cl:32-336) with the elitist copy of
  * gpu_perform_elitist_selection (auxiliary_genetic.cl:89-162). */
-void orc_gen(orc_job* j, int r0, int r1) {
+static void gen_one(orc_job* j, int r, int e) {
     const int pop = j->sp.pop_size;
     const int NG = j->lig->num_rotbonds + 6;
     const float tr = j->sp.tournament_rate / 100.0f, cr = j->sp.crossover_rate / 100.0f;
     const float mr = j->sp.mutation_rate / 100.f;
-    for (int r = r0; r < r1; ++r) {
-        for (int e = 0; e < pop; ++e) {
+    {
+        {
             const int b = r * pop + e;
             uint32_t* st = j->prng + (size_t)b * ADGPU_NWI;   /* thread tidx uses st[tidx] */
             float* child = j->pop_next + (size_t)b * 64;
             if (e == 0) {
                 /* elitism: the NWI-strided scan keeps, per work-item, the first
                  * strict minimum; work-items are then scanned in order. */
-                float best[ADGPU_NWI]; int bid[ADGPU_NWI];
+                float best[ADGPU_NWI] = {0}; int bid[ADGPU_NWI] = {0};
                 int nl = pop < ADGPU_NWI ? pop : ADGPU_NWI;
                 for (int t = 0; t < nl; ++t) { best[t] = j->ene_cur[r * pop + t]; bid[t] = t; }
                 for (int t = 0; t < nl; ++t)
@@ -742,7 +791,7 @@ void orc_gen(orc_job* j, int r0, int r1) {
                 j->ene_next[b] = be;
                 j->evals_new[b] = 0;
                 for (int k = 0; k < NG; ++k) child[k] = j->pop_cur[(size_t)(r * pop + bi) * 64 + k];
-                continue;
+                return;
             }
             float rn[10];
             for (int k = 0; k < 10; ++k) rn[k] = orc_randf(&st[k]);
@@ -775,6 +824,11 @@ void orc_gen(orc_job* j, int r0, int r1) {
             j->evals_new[b] = 1;
         }
     }
+}
+
+void orc_gen(orc_job* j, int r0, int r1) {
+    for (int r = r0; r < r1; ++r)
+        for (int e = 0; e < j->sp.pop_size; ++e) gen_one(j, r, e);
 }
 
 static int ls_entities(const orc_job* j) {
@@ -966,21 +1020,54 @@ static void ls_fire(orc_job* j, int r, int le) {
 }
 
 /* LS dispatch (performdocking.cpp.Cuda:727-785; OpenCL host adds FIRE). */
-void orc_ls(orc_job* j, int r0, int r1, int64_t gen) {
-    if (j->sp.lsearch_rate == 0.0f) return;
-    const int nls = ls_entities(j);
+static int ls_method_of(const orc_job* j, int64_t gen) {
     int method = j->sp.ls_method;
     if (method == ADGPU_LS_AD && gen < j->sp.initial_sw_generations) method = ADGPU_LS_SW;
+    return method;
+}
+
+static void ls_one(orc_job* j, int r, int le, int method) {
+    switch (method) {
+        case ADGPU_LS_SW:   ls_sw(j, r, le); break;
+        case ADGPU_LS_AD:   ls_ad(j, r, le); break;
+        case ADGPU_LS_ADAM: ls_adam(j, r, le); break;
+        case ADGPU_LS_FIRE: ls_fire(j, r, le); break;
+        default: break;
+    }
+}
+
+void orc_ls(orc_job* j, int r0, int r1, int64_t gen) {
+    if (j->sp.lsearch_rate == 0.0f) return;
+    const int nls = ls_entities(j), method = ls_method_of(j, gen);
     for (int r = r0; r < r1; ++r)
-        for (int le = 0; le < nls; ++le) {
-            switch (method) {
-                case ADGPU_LS_SW:   ls_sw(j, r, le); break;
-                case ADGPU_LS_AD:   ls_ad(j, r, le); break;
-                case ADGPU_LS_ADAM: ls_adam(j, r, le); break;
-                case ADGPU_LS_FIRE: ls_fire(j, r, le); break;
-                default: break;
-            }
+        for (int le = 0; le < nls; ++le) ls_one(j, r, le, method);
+}
+
+/* One generation of runs [r0, r1) split over nthreads threads by
+ * (run, individual) -- the work-groups the reference's OpenCL CPU runtime
+ * schedules (one per individual in kernel4, one per LS entity in the local
+ * search).  Every individual and every search only touches its own entity,
+ * so the result does not depend on the split; the one exception is the
+ * clamped entity choice when all individuals are searched at a rate below
+ * 100 % (ls_pick), where each run's searches then stay in order. */
+static void generation_split(orc_job* j, int r0, int r1, int64_t gen, int nthreads) {
+    const int pop = j->sp.pop_size;
+    const int64_t ng = (int64_t)(r1 - r0) * pop;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 8)
+    for (int64_t k = 0; k < ng; ++k) gen_one(j, r0 + (int)(k / pop), (int)(k % pop));
+    if (j->sp.lsearch_rate != 0.0f) {
+        const int nls = ls_entities(j), method = ls_method_of(j, gen);
+        if (nls >= pop && j->sp.lsearch_rate < 100.0f) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+            for (int r = r0; r < r1; ++r)
+                for (int le = 0; le < nls; ++le) ls_one(j, r, le, method);
+        } else {
+            const int64_t nl = (int64_t)(r1 - r0) * nls;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+            for (int64_t k = 0; k < nl; ++k) ls_one(j, r0 + (int)(k / nls), (int)(k % nls), method);
         }
+    }
+    orc_sum_evals(j, r0, r1);
 }
 
 /* check_progress (performdocking.cpp.Cuda:908-942) */
@@ -995,20 +1082,14 @@ static double check_progress(const orc_job* j, int64_t gen, int64_t* total) {
 
 int64_t orc_lga_run(orc_job* j, int nthreads, int64_t* total_evals) {
     const int R_ = j->sp.num_runs;
-    (void)nthreads;
     /* runs are independent except for the shared stopping test, which reads
      * the totals after every generation; the generation loop therefore stays
-     * global and each generation's work is split over runs. */
+     * global and each generation's work is split over (run, individual). */
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int r = 0; r < R_; ++r) orc_initpop(j, r, r + 1);
     int64_t gen = 0, tot = 0;
     while (check_progress(j, gen, &tot) < 100.0) {
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
-        for (int r = 0; r < R_; ++r) {
-            orc_gen(j, r, r + 1);
-            orc_ls(j, r, r + 1, gen);
-            orc_sum_evals(j, r, r + 1);
-        }
+        generation_split(j, 0, R_, gen, nthreads);
         gen++;
         orc_swap(j);
     }
@@ -1022,12 +1103,7 @@ int64_t orc_lga_generations(orc_job* j, int r0, int r1, int64_t ngen, int nthrea
     int64_t before = 0, after = 0;
     for (int r = r0; r < r1; ++r) before += j->evals_runs[r];
     for (int64_t g = 0; g < ngen; ++g) {
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
-        for (int r = r0; r < r1; ++r) {
-            orc_gen(j, r, r + 1);
-            orc_ls(j, r, r + 1, j->generation + g);
-            orc_sum_evals(j, r, r + 1);
-        }
+        generation_split(j, r0, r1, j->generation + g, nthreads);
         orc_swap(j);
     }
     j->generation += ngen;

@@ -58,6 +58,32 @@ void   orc_gradient_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_
 void   orc_coords_f32(const adgpu_ligand* L, const float* genotype, float* xyz);
 void   orc_coords_f64(const adgpu_ligand* L, const float* genotype, double* xyz);
 
+/* Batches of n genotypes (stride 64) over nthreads threads: energies, their
+ * intermolecular part and sums of |term|, merged energy + gradient (grads:
+ * n*64), float gradient (n*64), energy sensitivity (as orc_atom_grad).
+ * Optional outputs may be NULL (not grads of orc_gradient_batch or
+ * sensitivity). */
+void orc_energy_batch_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                          const float* genotypes, int64_t n, float* energies, float* inters, float* abs_sums,
+                          int nthreads);
+void orc_energy_batch_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                          const float* genotypes, int64_t n, double* energies, double* inters, double* abs_sums,
+                          int nthreads);
+void orc_energrad_batch_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                            const float* genotypes, int64_t n, int quantize, float* energies, float* grads,
+                            int nthreads);
+void orc_energrad_batch_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                            const float* genotypes, int64_t n, int quantize, double* energies, double* grads,
+                            int nthreads);
+void orc_gradient_batch_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                            const float* genotypes, int64_t n, float* grads, int nthreads);
+void orc_gradient_batch_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                            const float* genotypes, int64_t n, double* grads, int nthreads);
+void orc_sensitivity_batch_f32(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                               const float* genotypes, int64_t n, float* sensitivity, int nthreads);
+void orc_sensitivity_batch_f64(const adgpu_grid* g, const adgpu_ligand* L, const adgpu_energy_params* ep,
+                               const float* genotypes, int64_t n, double* sensitivity, int nthreads);
+
 /* fast_acos (calcenergy.cl:93-108), the polynomial arc cosine of the
  * orientation gradient. */
 float  orc_fast_acos_f32(float cosine);

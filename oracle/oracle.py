@@ -64,6 +64,11 @@ def lib():
         L.orc_atom_grad_f64.argtypes = [gp, lp, ep, _fp, _dp, _dp]
         L.orc_gradient_f32.argtypes = [gp, lp, ep, _fp, _fp]
         L.orc_gradient_f64.argtypes = [gp, lp, ep, _fp, _dp]
+        for t, pt in (("f32", _fp), ("f64", _dp)):
+            getattr(L, "orc_energy_batch_" + t).argtypes = [gp, lp, ep, _fp, C.c_int64, pt, pt, pt, C.c_int]
+            getattr(L, "orc_energrad_batch_" + t).argtypes = [gp, lp, ep, _fp, C.c_int64, C.c_int, pt, pt, C.c_int]
+            getattr(L, "orc_gradient_batch_" + t).argtypes = [gp, lp, ep, _fp, C.c_int64, pt, C.c_int]
+            getattr(L, "orc_sensitivity_batch_" + t).argtypes = [gp, lp, ep, _fp, C.c_int64, pt, C.c_int]
         L.orc_fast_acos_f64.restype = C.c_double
         L.orc_fast_acos_f64.argtypes = [C.c_double]
         L.orc_coords_f32.argtypes = [lp, _fp, _fp]
@@ -139,6 +144,53 @@ class Oracle:
         s = C.c_double()
         lib().orc_atom_grad_f64(*self._args(), g.ctypes.data_as(_fp), None, C.byref(s))
         return float(s.value)
+
+    # batches of poses (rows of 64 floats), split over host threads
+    @staticmethod
+    def _threads(nthreads):
+        return int(nthreads or min(16, os.cpu_count() or 1))
+
+    def energy_batch(self, genos, f64=False, nthreads=0):
+        """Energies, their intermolecular part and sums of |term| of every pose."""
+        g = _f(genos).reshape(-1, 64)
+        n = g.shape[0]
+        dt, pt, t = (np.float64, _dp, "f64") if f64 else (np.float32, _fp, "f32")
+        e, i, a = np.zeros(n, dt), np.zeros(n, dt), np.zeros(n, dt)
+        getattr(lib(), "orc_energy_batch_" + t)(*self._args(), g.ctypes.data_as(_fp), n, e.ctypes.data_as(pt),
+                                                i.ctypes.data_as(pt), a.ctypes.data_as(pt), self._threads(nthreads))
+        return e, i, a
+
+    def energrad_batch(self, genos, f64=False, quantize=None, nthreads=0):
+        """Merged energy and gradient (n, 64) of every pose; FP32 quantizes
+        like the device unless told otherwise, FP64 does not."""
+        g = _f(genos).reshape(-1, 64)
+        n = g.shape[0]
+        dt, pt, t = (np.float64, _dp, "f64") if f64 else (np.float32, _fp, "f32")
+        q = (not f64) if quantize is None else bool(quantize)
+        e, gr = np.zeros(n, dt), np.zeros((n, 64), dt)
+        getattr(lib(), "orc_energrad_batch_" + t)(*self._args(), g.ctypes.data_as(_fp), n, int(q),
+                                                  e.ctypes.data_as(pt), gr.ctypes.data_as(pt),
+                                                  self._threads(nthreads))
+        return e, gr
+
+    def gradient_batch(self, genos, f64=False, nthreads=0):
+        """Float gradient (FIRE's first step), (n, 64)."""
+        g = _f(genos).reshape(-1, 64)
+        n = g.shape[0]
+        dt, pt, t = (np.float64, _dp, "f64") if f64 else (np.float32, _fp, "f32")
+        gr = np.zeros((n, 64), dt)
+        getattr(lib(), "orc_gradient_batch_" + t)(*self._args(), g.ctypes.data_as(_fp), n, gr.ctypes.data_as(pt),
+                                                  self._threads(nthreads))
+        return gr
+
+    def sensitivity_batch(self, genos, nthreads=0):
+        """sum_a |dE/dx_a| |x_a| of every pose (FP64)."""
+        g = _f(genos).reshape(-1, 64)
+        n = g.shape[0]
+        s = np.zeros(n, np.float64)
+        lib().orc_sensitivity_batch_f64(*self._args(), g.ctypes.data_as(_fp), n, s.ctypes.data_as(_dp),
+                                        self._threads(nthreads))
+        return s
 
     def atom_grad64(self, geno):
         """Per-atom dE/dx (kcal/mol/A, unquantized), shape (atoms, 3)."""
