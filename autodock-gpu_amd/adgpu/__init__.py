@@ -129,6 +129,12 @@ class Context:
                                               e.ctypes.data_as(_fp), gr.ctypes.data_as(_fp)), "adgpu_eval_poses")
         return e, gr
 
+    def measure_valu_rate(self):
+        """Wavefront VALU instructions per second on independent FP32 FMA chains."""
+        r = C.c_double()
+        self._check(self.lib.adgpu_measure_valu_rate(self.h, C.byref(r)), "adgpu_measure_valu_rate")
+        return r.value
+
     def job(self, map_id, ligand_id, sp, init_pop, seeds) -> "Job":
         return Job(self, map_id, ligand_id, sp, init_pop, seeds)
 
@@ -230,6 +236,12 @@ class Job:
         self._c(self.lib.adgpu_job_kernel_busy(self.h, cls, C.byref(ms), C.byref(n)), "adgpu_job_kernel_busy")
         return ms.value, n.value
 
+    def ls_handoffs(self):
+        """ADADELTA searches handed off between teams since the job started."""
+        n = C.c_int64()
+        self._c(self.lib.adgpu_job_ls_handoffs(self.h, C.byref(n)), "adgpu_job_ls_handoffs")
+        return n.value
+
     def set_autostop(self, enable=True, stopstd=0.15, as_frequency=5, min_as_evals=0):
         """AutoStop tested on the device every as_frequency generations."""
         self._c(self.lib.adgpu_job_set_autostop(self.h, int(bool(enable)), float(stopstd), int(as_frequency),
@@ -255,8 +267,9 @@ class Job:
 
 class Server:
     """Serving loop (adgpu_server_*): docking requests run concurrently in
-    `slots` slots, admitted in arrival order when a slot is free and their
-    device memory fits `memory_budget` bytes (0 = 90 % of free memory)."""
+    `slots` slots, admitted in arrival order when a slot is free and the
+    device memory every slot's buffers hold, idle slots included, fits
+    `memory_budget` bytes (0 = 90 % of free memory)."""
 
     def __init__(self, ctx: Context, slots: int = 8, run_groups: int = 0, memory_budget: int = 0):
         self.ctx, self.lib = ctx, ctx.lib

@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(HERE, "libadgpu_hip.so")
 
 # one translation unit per kernel family, compiled in parallel
-SOURCES = ["k_ls_ad.hip", "k_ls_sw.hip", "k_ls_adam.hip", "k_ls_fire.hip", "k_ga.hip", "k_eval.hip", "k_rescore.hip", "k_autostop.hip",
+SOURCES = ["k_ls_ad.hip", "k_ls_sw.hip", "k_ls_adam.hip", "k_ls_fire.hip", "k_ga.hip", "k_eval.hip", "k_rescore.hip", "k_autostop.hip", "k_calib.hip",
            "adgpu_host.cpp", "adgpu_serve.cpp"]
 ARCH = os.environ.get("ADGPU_OFFLOAD_ARCH", "gfx950")
 

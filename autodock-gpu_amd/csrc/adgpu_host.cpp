@@ -967,8 +967,39 @@ static int check_handoff(adgpu_job_t j) {
     if (j->sd.ls_yield <= 0) return ADGPU_OK;
     int32_t q[16 * kMaxGroups];
     HIPCHK(j->ctx, hipMemcpy(q, j->counters, sizeof(q), hipMemcpyDeviceToHost));
-    for (int k = 0; k < kMaxGroups; ++k)
+    for (int k = 0; k < kMaxGroups; ++k) {
+        if (q[16 * k + 3] == 2)
+            return fail(j->ctx, ADGPU_ERR_HIP, "ADADELTA hand-off: a saved search record was invalid when resumed");
         if (q[16 * k + 3]) return fail(j->ctx, ADGPU_ERR_HIP, "ADADELTA hand-off between teams did not complete");
+    }
+    return ADGPU_OK;
+}
+
+int adgpu_measure_valu_rate(adgpu_ctx c, double* wave_instr_per_s) {
+    if (!c || !wave_instr_per_s) return fail(c, ADGPU_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    int cus = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int blocks = std::max(1, cus) * 8;   // 32 wavefronts per compute unit, 8 per SIMD
+    float* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)blocks * 256 * sizeof(float)) != hipSuccess)
+        return fail(c, ADGPU_ERR_NOMEM, "hipMalloc(calibration) failed");
+    const hipError_t e = measure_valu_rate(c->stream, buf, blocks, 20000, wave_instr_per_s);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(c, ADGPU_ERR_HIP, std::string("VALU rate: ") + hipGetErrorString(e));
+    return ADGPU_OK;
+}
+
+int adgpu_job_ls_handoffs(adgpu_job_t j, int64_t* count) {
+    if (!j || !count) return ADGPU_ERR_ARG;
+    HIPCHK(j->ctx, hipSetDevice(j->ctx->device));
+    for (auto& g : j->groups) HIPCHK(j->ctx, hipStreamSynchronize(g.stream));
+    int32_t q[16 * kMaxGroups];
+    HIPCHK(j->ctx, hipMemcpy(q, j->counters, sizeof(q), hipMemcpyDeviceToHost));
+    int64_t n = 0;
+    for (int k = 0; k < kMaxGroups; ++k) n += q[16 * k + 4];
+    *count = n;
     return ADGPU_OK;
 }
 
@@ -1481,26 +1512,99 @@ int map_hold(adgpu_context* c, int map_id, int delta) {
     return ADGPU_OK;
 }
 
-size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params* sp, int as_frequency) {
-    const size_t nrun = (size_t)sp->num_runs, ne = nrun * (size_t)sp->pop_size;
-    size_t b = 2 * ne * kGenes * sizeof(float) + 2 * ne * sizeof(float) + ne * sizeof(int32_t) +
-               ne * ADGPU_NWI * sizeof(uint32_t) + nrun * sizeof(int32_t) + lig.blob.size() +
-               nrun * (kGenes * sizeof(float) + 3 * sizeof(double));   // re-scoring scratch
-    if (sp->ls_method == ADGPU_LS_FIRE) {
-        const size_t nls = (size_t)std::max(1, to_dev(sp, lig.hd.num_genes).num_ls);
-        b += nrun * nls * (sizeof(int32_t) + kGenes * sizeof(float));
-    }
-    if (sp->ls_method == ADGPU_LS_AD) {   // saved searches and hand-off ring (when hand-off is on)
-        const size_t nls = (size_t)std::max(1, to_dev(sp, lig.hd.num_genes).num_ls);
-        b += nrun * nls * (kAdStateFloats * sizeof(float) + sizeof(int32_t));
-    }
-    if (as_frequency > 0) {   // log_cap as adgpu_job_set_autostop sizes it
-        const int64_t tests = sp->max_generations / as_frequency + 2;
-        const size_t cap = (size_t)std::max<int64_t>(16, std::min<int64_t>(tests, 16384));
-        b += sizeof(AutoStopDev) + 3 * ((size_t)sp->pop_size + 1) * sizeof(float) +
-             (cap + 1) * sizeof(adgpu_autostop_record);
-    }
+// Capacities of a slot's size-dependent device buffers: what it holds now,
+// or what it holds after binding a request (each buffer only grows, so the
+// component-wise maximum of both).  The sizes follow job_reserve, slot_bind,
+// adgpu_job_set_autostop and slot_results.
+struct SlotCaps {
+    size_t ne = 0, ls = 0, yield = 0, own = 0, as_bins = 0;
+    int runs = 0, as_log = 0, best = 0;
+    bool as = false;   // AutoStop state and final record
+};
+
+static size_t caps_bytes(const SlotCaps& k) {
+    size_t b = k.ne * (2 * kGenes * sizeof(float) + 2 * sizeof(float) + sizeof(int32_t) +
+                       ADGPU_NWI * sizeof(uint32_t));
+    b += (size_t)k.runs * sizeof(int32_t);
+    b += k.ls * (sizeof(int32_t) + kGenes * sizeof(float));
+    b += k.yield * (kAdStateFloats * sizeof(float) + sizeof(int32_t));
+    b += k.own;
+    if (k.as) b += sizeof(AutoStopDev) + sizeof(adgpu_autostop_record);
+    b += k.as_bins * sizeof(float) + (size_t)k.as_log * sizeof(adgpu_autostop_record);
+    b += (size_t)k.best * (kGenes * sizeof(float) + 3 * sizeof(double));
     return b;
+}
+
+static SlotCaps caps_of(const adgpu_job* j) {
+    SlotCaps k;
+    k.ne = j->cap_ne; k.ls = j->cap_ls; k.yield = j->cap_yield; k.own = j->own_cap;
+    k.as_bins = j->as_bins_cap; k.runs = j->cap_runs; k.as_log = j->as_log_alloc; k.best = j->best_cap;
+    k.as = j->d_as != nullptr;
+    return k;
+}
+
+static SlotCaps caps_needed(const adgpu_context* c, const PreparedLigand& lig, const adgpu_search_params* sp,
+                            int as_frequency, bool rescore) {
+    SlotCaps k;
+    const int nrun = sp->num_runs;
+    const size_t nls_all = (size_t)nrun * std::max(1, to_dev(sp, lig.hd.num_genes).num_ls);
+    k.ne = (size_t)nrun * sp->pop_size;
+    k.runs = nrun;
+    k.ls = sp->ls_method == ADGPU_LS_FIRE ? nls_all : 0;
+    k.yield = yield_wanted(c, sp) ? nls_all : 0;
+    k.own = (lig.blob.size() + 65535) & ~size_t(65535);
+    if (as_frequency > 0) {
+        k.as = true;
+        k.as_bins = 3 * ((size_t)sp->pop_size + 1);
+        const int64_t tests = sp->max_generations / as_frequency + 2;
+        k.as_log = (int)std::max<int64_t>(16, std::min<int64_t>(tests, 16384));
+    }
+    k.best = rescore ? nrun : 0;
+    return k;
+}
+
+static SlotCaps caps_max(const SlotCaps& a, const SlotCaps& b) {
+    SlotCaps k;
+    k.ne = std::max(a.ne, b.ne); k.ls = std::max(a.ls, b.ls); k.yield = std::max(a.yield, b.yield);
+    k.own = std::max(a.own, b.own); k.as_bins = std::max(a.as_bins, b.as_bins);
+    k.runs = std::max(a.runs, b.runs); k.as_log = std::max(a.as_log, b.as_log); k.best = std::max(a.best, b.best);
+    k.as = a.as || b.as;
+    return k;
+}
+
+size_t request_device_bytes(const adgpu_context* c, const PreparedLigand& lig, const adgpu_search_params* sp,
+                            int as_frequency, bool rescore) {
+    return caps_bytes(caps_needed(c, lig, sp, as_frequency, rescore));
+}
+
+size_t slot_held_bytes(const adgpu_job* j) { return caps_bytes(caps_of(j)); }
+
+size_t slot_bytes_after(const adgpu_job* j, const PreparedLigand& lig, const adgpu_search_params* sp,
+                        int as_frequency, bool rescore) {
+    return caps_bytes(caps_max(caps_of(j), caps_needed(j->ctx, lig, sp, as_frequency, rescore)));
+}
+
+void slot_shrink(adgpu_job* j) {
+    (void)hipSetDevice(j->ctx->device);
+    for (auto& g : j->groups) (void)hipStreamSynchronize(g.stream);
+    (void)hipFree(j->pop[0]); (void)hipFree(j->pop[1]);
+    (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
+    (void)hipFree(j->evals_new); (void)hipFree(j->prng); (void)hipFree(j->evals_runs);
+    (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
+    (void)hipFree(j->ls_state); (void)hipFree(j->ls_ring);
+    (void)hipFree(j->own.dev);
+    (void)hipFree(j->d_as); (void)hipFree(j->d_as_out); (void)hipFree(j->d_as_bins); (void)hipFree(j->d_as_log);
+    (void)hipFree(j->d_best); (void)hipFree(j->d_f64);
+    if (j->h_ring) (void)hipHostFree(j->h_ring);
+    j->pop[0] = j->pop[1] = j->ene[0] = j->ene[1] = nullptr;
+    j->evals_new = nullptr; j->prng = nullptr; j->evals_runs = nullptr; j->h_ring = nullptr;
+    j->ls_ent = nullptr; j->ls_grad = nullptr; j->ls_state = nullptr; j->ls_ring = nullptr;
+    j->own.dev = nullptr;
+    j->d_as = nullptr; j->d_as_out = nullptr; j->d_as_bins = nullptr; j->d_as_log = nullptr;
+    j->d_best = nullptr; j->d_f64 = nullptr;
+    j->cap_ne = j->cap_ls = j->cap_yield = j->own_cap = j->as_bins_cap = 0;
+    j->cap_runs = j->as_log_alloc = j->best_cap = 0;
+    j->as_enabled = false;
 }
 
 int slot_open(adgpu_context* c, int run_groups, int sharers, adgpu_job** out) {

@@ -140,6 +140,10 @@ hipError_t launch_ls_fire(const LaunchArgs& la, const SearchDev& sp, const JobBu
 hipError_t launch_autostop(hipStream_t s, const AutoStopParams& p, const float* ene, const int32_t* evals_runs,
                            AutoStopDev* st, float* bins, adgpu_autostop_record* log, int32_t* stop,
                            int64_t generation, int mode, adgpu_autostop_record* out);
+// Wavefront VALU instructions per second of a kernel of independent FP32
+// FMA chains over `blocks` workgroups of 256 threads (k_calib.hip); scratch
+// holds blocks * 256 floats.
+hipError_t measure_valu_rate(hipStream_t stream, float* scratch, int blocks, int iters, double* wave_instr_per_s);
 // LDS bytes and team width the launchers use for a ligand.
 size_t lds_bytes(const LigandDev& lig_host, int team_hint, int* team_width);
 

@@ -1,17 +1,21 @@
 // adgpu_serve.cpp -- the serving loop of include/adgpu.h (adgpu_server_*).
 //
 // Requests queue in arrival order.  Each slot has one worker thread and one
-// reusable job (adgpu_slots.h); a free worker admits the head of the queue
-// when the request's device bytes fit the budget beside the running ones,
-// runs the whole docking (initial population, generation loop with
-// AutoStop, per-run best and optional FP64 re-scoring) on its own streams,
-// and files the result.  The reference's counterpart is the per-ligand loop
+// reusable job (adgpu_slots.h) whose device buffers outlive its request.  The
+// memory budget bounds what the slots hold, idle ones included: the head of
+// the queue goes to the idle slot that needs the least growth to bind it,
+// when the bytes of every slot after that growth fit the budget; when no idle
+// slot fits but freeing the buffers of idle slots would make room, those
+// slots free them first.  A worker runs the whole docking (initial
+// population, generation loop with AutoStop, per-run best and optional FP64
+// re-scoring) on its own streams and files the result.  The reference's counterpart is the per-ligand loop
 // of main.cpp (:312-327,401-434), which docks one ligand per device at a time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -64,10 +68,14 @@ struct adgpu_server {
     std::map<int64_t, std::unique_ptr<Request>> reqs;
     std::deque<int64_t> finished;      // finished and not yet returned by next_done
     int64_t next_ticket = 1, submitted = 0, completed = 0, failed = 0;
-    size_t in_use = 0, peak_bytes = 0;
+    size_t in_use = 0, peak_bytes = 0;   // device bytes the slots hold (sum of held)
     int active = 0, peak_active = 0;
     bool stopping = false;
     std::vector<adgpu_job*> slots;
+    std::vector<size_t> held;            // per slot: bytes of its buffers (after growth, once assigned)
+    std::vector<Request*> assigned;      // per slot: request handed to it, not yet taken by its worker
+    std::vector<char> busy;              // per slot: assigned or running a request
+    std::vector<char> shrink;            // per slot: idle, asked to free its buffers
     std::vector<std::thread> workers;
 
     double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
@@ -107,6 +115,67 @@ void run_request(adgpu_server* s, adgpu_job* j, Request* r) {
     (void)s;
 }
 
+bool rescore_of(const Request* r) { return (r->flags & ADGPU_REQ_RESCORE_F64) != 0; }
+
+// Admission, with s->mu held: hands queued requests, in arrival order, to idle
+// slots while their bytes fit, or asks idle slots to free their buffers when
+// only that makes room for the head.  Returns true when a worker has work.
+bool schedule(adgpu_server* s) {
+    bool work = false;
+    while (!s->queue.empty()) {
+        Request* r = s->queue.front();
+        const int as_freq = r->autostop ? r->as_frequency : 0;
+        int best = -1;
+        size_t best_after = 0, best_growth = SIZE_MAX;
+        for (size_t k = 0; k < s->slots.size(); ++k) {
+            if (s->busy[k] || s->shrink[k]) continue;
+            const size_t after = slot_bytes_after(s->slots[k], r->lig, &r->sp, as_freq, rescore_of(r));
+            if (s->in_use - s->held[k] + after > s->budget) continue;
+            if (after - s->held[k] < best_growth) {
+                best = (int)k;
+                best_after = after;
+                best_growth = after - s->held[k];
+            }
+        }
+        if (best >= 0) {
+            s->queue.pop_front();
+            s->in_use += best_after - s->held[best];
+            s->held[best] = best_after;
+            s->peak_bytes = std::max(s->peak_bytes, s->in_use);
+            s->busy[best] = 1;
+            s->assigned[best] = r;
+            s->active++;
+            s->peak_active = std::max(s->peak_active, s->active);
+            r->res.t_admit_ms = s->now_ms();
+            r->res.slot = best;
+            work = true;
+            continue;
+        }
+        // No idle slot fits.  Freeing idle slots' buffers makes room for the
+        // head (in an emptied slot it needs r->bytes) if the bytes held by
+        // running slots leave that much; free the largest first.
+        size_t running = 0, releasing = 0;
+        std::vector<int> idle;
+        for (size_t k = 0; k < s->slots.size(); ++k) {
+            if (s->busy[k]) running += s->held[k];
+            else if (s->shrink[k]) releasing += s->held[k];
+            else if (s->held[k] > 0) idle.push_back((int)k);
+        }
+        if (running + r->bytes <= s->budget && !idle.empty()) {
+            std::sort(idle.begin(), idle.end(), [&](int a, int b) { return s->held[a] > s->held[b]; });
+            size_t freed = releasing;
+            for (int k : idle) {
+                if (s->in_use - freed + r->bytes <= s->budget) break;
+                s->shrink[k] = 1;
+                freed += s->held[k];
+                work = true;
+            }
+        }
+        break;   // the head waits for running requests or for buffers being freed
+    }
+    return work;
+}
+
 void worker(adgpu_server* s, int slot) {
     (void)hipSetDevice(s->device);
     adgpu_job* j = s->slots[slot];
@@ -114,20 +183,24 @@ void worker(adgpu_server* s, int slot) {
         Request* r = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] {
-                return s->stopping || (!s->queue.empty() && s->in_use + s->queue.front()->bytes <= s->budget);
-            });
+            s->cv_work.wait(lk, [&] { return s->stopping || s->assigned[slot] || s->shrink[slot]; });
             if (s->stopping) return;
-            r = s->queue.front();
-            s->queue.pop_front();
-            s->in_use += r->bytes;
-            s->peak_bytes = std::max(s->peak_bytes, s->in_use);
-            s->active++;
-            s->peak_active = std::max(s->peak_active, s->active);
-            r->res.t_admit_ms = s->now_ms();
-            r->res.slot = slot;
+            r = s->assigned[slot];
+            s->assigned[slot] = nullptr;
         }
-        s->cv_work.notify_one();   // the next request may fit beside this one
+        if (!r) {   // asked to free this idle slot's buffers
+            slot_shrink(j);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->in_use -= s->held[slot];
+                s->held[slot] = slot_held_bytes(j);
+                s->in_use += s->held[slot];
+                s->shrink[slot] = 0;
+                schedule(s);
+            }
+            s->cv_work.notify_all();
+            continue;
+        }
         try {
             run_request(s, j, r);
         } catch (const std::exception& ex) {   // host memory for the results, for instance
@@ -138,13 +211,19 @@ void worker(adgpu_server* s, int slot) {
         (void)map_hold(s->ctx, r->map_id, -1);
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->in_use -= r->bytes;
+            // what the slot keeps for its next request (a failed growth may
+            // have left less than was charged)
+            s->in_use -= s->held[slot];
+            s->held[slot] = slot_held_bytes(j);
+            s->in_use += s->held[slot];
+            s->busy[slot] = 0;
             s->active--;
             r->res.t_done_ms = s->now_ms();
             r->done = true;
             s->completed++;
             if (r->res.status) s->failed++;
             s->finished.push_back(r->ticket);
+            schedule(s);
         }
         s->cv_work.notify_all();
         s->cv_done.notify_all();
@@ -191,6 +270,10 @@ int adgpu_server_create(adgpu_ctx ctx, const adgpu_server_config* cfg, adgpu_ser
         }
         s->slots.push_back(j);
     }
+    s->held.assign(s->slots.size(), 0);
+    s->assigned.assign(s->slots.size(), nullptr);
+    s->busy.assign(s->slots.size(), 0);
+    s->shrink.assign(s->slots.size(), 0);
     for (int k = 0; k < cfg->slots; ++k) s->workers.emplace_back(worker, s.get(), k);
     *out = s.release();
     return ADGPU_OK;
@@ -215,7 +298,7 @@ int adgpu_server_submit(adgpu_server_t s, const adgpu_request* q, int64_t* ticke
     r->stopstd = q->stopstd;
     r->min_as_evals = q->min_as_evals;
     r->flags = q->flags;
-    r->bytes = request_device_bytes(r->lig, &r->sp, r->autostop ? r->as_frequency : 0);
+    r->bytes = request_device_bytes(s->ctx, r->lig, &r->sp, r->autostop ? r->as_frequency : 0, rescore_of(r.get()));
     r->res.num_runs = r->sp.num_runs;
     r->res.pop_size = r->sp.pop_size;
     r->res.device_bytes = (int64_t)r->bytes;
@@ -234,8 +317,9 @@ int adgpu_server_submit(adgpu_server_t s, const adgpu_request* q, int64_t* ticke
         s->queue.push_back(r.get());
         s->reqs[r->ticket] = std::move(r);
         s->submitted++;
+        schedule(s);
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return ADGPU_OK;
 }
 
@@ -309,6 +393,8 @@ int adgpu_server_destroy(adgpu_server_t s) {
         s->stopping = true;
         for (Request* r : s->queue) (void)map_hold(s->ctx, r->map_id, -1);   // dropped unrun
         s->queue.clear();
+        for (Request*& r : s->assigned)
+            if (r) { (void)map_hold(s->ctx, r->map_id, -1); r = nullptr; }    // admitted, not started
     }
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();

@@ -32,10 +32,19 @@ int check_request(adgpu_context* c, int map_id, const PreparedLigand& lig, const
 // Adds delta to the uses of a map set (a queued or running request holds
 // one, so the map set cannot be released under it).
 int map_hold(adgpu_context* c, int map_id, int delta);
-// Device bytes one request occupies in a slot (population buffers, PRNG
-// states, ligand block, AutoStop state when as_frequency > 0, re-scoring
-// scratch).
-size_t request_device_bytes(const PreparedLigand& lig, const adgpu_search_params* sp, int as_frequency);
+// Device bytes one request occupies in an empty slot (population buffers,
+// PRNG states, LS state, ligand block, AutoStop state when as_frequency > 0,
+// re-scoring scratch when rescore).
+size_t request_device_bytes(const adgpu_context* c, const PreparedLigand& lig, const adgpu_search_params* sp,
+                            int as_frequency, bool rescore);
+// Device bytes a slot's size-dependent buffers hold now (they outlive the
+// request, for the next one), and what they will hold once the slot has
+// bound the request (buffers only grow).
+size_t slot_held_bytes(const adgpu_job* j);
+size_t slot_bytes_after(const adgpu_job* j, const PreparedLigand& lig, const adgpu_search_params* sp,
+                        int as_frequency, bool rescore);
+// Frees an idle slot's size-dependent buffers (after which it holds 0 bytes).
+void slot_shrink(adgpu_job* j);
 
 // sharers: requests that may run on the device at the same time (the
 // server's slots); the job's team width counts their searches too.

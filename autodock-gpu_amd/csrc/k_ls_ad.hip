@@ -111,9 +111,22 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
                 if (v >= 0) {
                     t = v;   // the record's loads depend on v: they cannot pass the ring load
                     gi = ad_load<TW>(it, a.B.ls_state + (size_t)t * kAdStateFloats);
-                    may_yield = false;
-                    active = true;
-                    need = false;
+                    // a record is saved at exactly yq iterations, for an entity of its
+                    // slot's run: anything else is a stale or torn record, reported
+                    // through the fault flag instead of being resumed
+                    const int run = t / nls;
+                    const bool valid = it.iters == yq && gi >= run * pop && gi < (run + 1) * pop;
+                    if (valid) {
+                        may_yield = false;
+                        active = true;
+                        need = false;
+                    } else {
+                        if (tl == 0) __hip_atomic_store(&Q[3], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        it.cons_succ = -1;
+                        it.cons_fail = 0;
+                        active = false;
+                        need = true;
+                    }
                 } else {
                     active = false;
                     done = v != -1;
@@ -163,7 +176,10 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
                 // hand the search off at claim c (one entry per claim, so at most once per slot)
                 ad_save<TW>(it, gi, a.B.ls_state + (size_t)t * kAdStateFloats);
                 __builtin_amdgcn_s_waitcnt(kWaitStores);   // the record is written before the ring entry
-                if (tl == 0) __hip_atomic_store(&ring[c], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (tl == 0) {
+                    __hip_atomic_store(&ring[c], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    atomicAdd(&Q[4], 1);   // hand-offs of the job (not cleared per launch)
+                }
                 t = c;
                 const int run = t / nls, le = t - run * nls;
                 gi = run * pop + ls_entity<TW>(S, le, a.B.prng + (size_t)t * 64);

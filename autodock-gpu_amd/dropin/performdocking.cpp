@@ -435,6 +435,27 @@ int docking_with_gpu(const Gridinfo* mygrid, Dockpars* mypars, const Liganddata*
     }
     para_printf("\nDocking time %fs\n", elapsed_seconds(t2, t3));
 
+    // Each run's best pose (the individual process_result puts first, the
+    // lowest FP32 energy of the run) re-scored in FP64 on the GPU.  The dlg's
+    // energies still come from the reference's CPU re-scoring of every
+    // individual (processresult.cpp:430-526), whose intramolecular energy
+    // uses distance tables; these are the same terms evaluated analytically.
+    // For flexible residues the split between the columns differs from the
+    // dlg's (residue-ligand pairs count as intra here); the totals agree.
+    {
+        std::vector<float> best_geno((size_t)nrun * ADGPU_GENOTYPE_STRIDE), best_e(nrun);
+        std::vector<int32_t> best_i(nrun);
+        std::vector<double> e_tot(nrun), e_inter(nrun), e_intra(nrun);
+        check(adgpu_job_best_f64(job, best_geno.data(), best_e.data(), best_i.data(), e_tot.data(), e_inter.data(),
+                                 e_intra.data()), "adgpu_job_best_f64", ctx);
+        para_printf("\nFP64 re-scoring on the GPU of each run's best pose (kcal/mol):\n");
+        para_printf("    Run | Entity |  FP32 energy |   FP64 total |   FP64 inter |   FP64 intra\n");
+        for (int r = 0; r < nrun; ++r)
+            para_printf("FP64 %3d | %6d | %12.4f | %12.4f | %12.4f | %12.4f\n", r + 1, best_i[r] + 1,
+                        (double)best_e[r], e_tot[r], e_inter[r], e_intra[r]);
+        fflush(stdout);
+    }
+
     sim_state.myligand_reference = myligand_reference;
     sim_state.generation_cnt = (int)gens;
     sim_state.sec_per_run = ((float)(clock_stop_docking - clock_start_docking) / (float)CLOCKS_PER_SEC) /

@@ -55,6 +55,15 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 # profiles/k_ls_ad_c1_traffic.md.  Keyed by (maps, nrun, pop, ls).
 TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 486.0}
 
+# Wavefront instructions of the local-search kernel per local-search
+# evaluation (SQ_INSTS_VALU and SQ_INSTS_LDS summed over its dispatches, over
+# the LS evaluations of those dispatches), from a rocprofv3 --pmc pass over
+# this workload: profiles/k_ls_ad_c1_counters.md.  With the measured
+# evaluation rate they give the kernel's VALU issue rate, reported against the
+# device's VALU issue rate on independent FP32 FMA chains measured live
+# (adgpu_measure_valu_rate).  Keyed by (maps, nrun, pop, ls).
+LS_INSTR_PER_EVAL = {}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -95,7 +104,20 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy kernel (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only measure the device's VALU issue rate on independent FP32 FMA chains "
+                         "(adgpu_measure_valu_rate) and exit")
     return ap.parse_args()
+
+
+def valu_calibration():
+    """The VALU issue-rate calibration kernel alone (under rocprofv3 its
+    counters give the counter readings at the peak rate)."""
+    import adgpu
+    ctx = adgpu.Context(0)
+    r = [ctx.measure_valu_rate() for _ in range(3)]
+    ctx.close()
+    print(json.dumps({"valu_wave_instr_per_s": r}), flush=True)
 
 
 def copy_calibration():
@@ -121,8 +143,13 @@ def build_inputs():
 
 
 def initial_state(seed, nrun, pop, maps):
-    """Initial population and LCG seeds drawn the way docking_with_gpu does
-    (CMWC4096 seeded from the job seed), generated on the host."""
+    """Initial population and LCG seeds, generated on the host from numpy's
+    seeded generator: translations uniform in the box, orientations from
+    uniform random quaternions converted to (phi, theta, alpha) as
+    gen_initpop_and_reflig does (getparameters.cpp:2594-2619), torsions
+    uniform in [0, 360), and one LCG state per work-item.  The reference
+    draws the same quantities from its CMWC4096 generator seeded with the
+    job seed; the docking does not depend on which generator made them."""
     rng = np.random.default_rng(seed)
     n = nrun * pop
     init = np.zeros((n, 64), np.float32)
@@ -574,6 +601,9 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
+    if a.valu_calibration:
+        valu_calibration()
+        return
     os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     if a.gpus != 1:
         print("bench.py measures one GPU: the docking path shards by ligand or by run with no "
@@ -686,6 +716,16 @@ def main():
                      "bytes_per_eval": bytes_per_eval, "aggregate_achieved": aggregate,
                      "aggregate_frac": aggregate / HBM_PEAK_GBS},
     }
+    if tkey in LS_INSTR_PER_EVAL:
+        # VALU issue rate of the LS kernel while it runs (LS busy time) against
+        # the calibrated rate of full-rate FP32 VALU instructions
+        valu_per_eval, lds_per_eval = LS_INSTR_PER_EVAL[tkey]
+        peak = ctx.measure_valu_rate()
+        rate = valu_per_eval * ls_evals / (float(np.sum(ls_busy)) * 1e-3)
+        out["roofline"]["valu"] = {"achieved": rate, "peak": peak, "unit": "wave-instr/s", "frac": rate / peak,
+                                   "valu_instr_per_eval": valu_per_eval, "lds_instr_per_eval": lds_per_eval,
+                                   "source": "profiles/k_ls_ad_c1_counters.md (instructions per evaluation); "
+                                             "peak measured live by adgpu_measure_valu_rate"}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(maps, tables, ep, a.cpu_evals)
     ctx.close()

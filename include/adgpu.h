@@ -277,6 +277,18 @@ int  adgpu_job_kernel_time(adgpu_job_t job, int kernel_class, double* avg_ms, in
  * for callers that time the kernels of overlapping run groups. */
 int  adgpu_job_kernel_busy(adgpu_job_t job, int kernel_class, double* busy_ms, int64_t* launches);
 
+/* Searches the ADADELTA hand-off passed from one team to another since the
+ * job started (adgpu_set_ls_handoff).  No counterpart in the reference, where
+ * every search runs to its end in its own block (cuda/kernel_ad.cu:60-120):
+ * measurement, for callers and tests that check the hand-off took place. */
+int  adgpu_job_ls_handoffs(adgpu_job_t job, int64_t* count);
+
+/* Wavefront VALU instructions per second the device issues on independent
+ * full-rate FP32 FMA chains (v_fma_f32, 32 wavefronts per compute unit): the
+ * peak that a kernel's measured VALU instruction rate is reported against.
+ * No counterpart in the reference: measurement. */
+int  adgpu_measure_valu_rate(adgpu_ctx ctx, double* wave_instr_per_s);
+
 /* ---- AutoStop on the device -------------------------------------------------
  * The reference tests its population every as_frequency generations on the
  * host (AutoStop::check_if_satisfactory, host/inc/autostop.hpp:181-231, called
@@ -324,18 +336,20 @@ int  adgpu_job_autostop_final(adgpu_job_t job, adgpu_autostop_record* out);
  * 401-434; the OVERLAP pipeline only overlaps a ligand's setup with the
  * previous docking).  Here a fixed number of slots run requests at the same
  * time, each on its own streams, so that the generations of different
- * ligands fill the device together; a request is admitted, in arrival order,
- * when a slot is free and its device memory fits the budget.  A slot keeps
- * its buffers for the next request (no allocation in steady state), and a
- * request's result does not depend on what else runs: every search is
- * seeded by its own request. */
+ * ligands fill the device together.  A slot keeps its buffers for the next
+ * request (no allocation in steady state), and the memory budget bounds what
+ * all slots hold, idle ones included: a request is admitted, in arrival
+ * order, into the idle slot that needs the least growth to run it, when every
+ * slot's buffers after that growth fit the budget; idle slots free their
+ * buffers when only that makes room.  A request's result does not depend on
+ * what else runs: every search is seeded by its own request. */
 typedef struct adgpu_server* adgpu_server_t;
 
 typedef struct adgpu_server_config {
     int32_t slots;            /* requests on the device at once (1..256)             */
     int32_t run_groups;       /* streams per request (0 = 1; at most one per run)     */
-    int64_t memory_budget;    /* device bytes for admitted requests (0 = 90 % of the
-                                 free device memory when the server starts)           */
+    int64_t memory_budget;    /* device bytes the slots' buffers may hold (0 = 90 % of
+                                 the free device memory when the server starts)       */
 } adgpu_server_config;
 
 #define ADGPU_REQ_KEEP_POPULATION  1   /* keep the final populations and energies   */
@@ -364,7 +378,7 @@ typedef struct adgpu_request_result {
     int32_t num_runs, pop_size;
     int64_t generations;      /* generation_cnt                                      */
     int64_t total_evals;      /* sum of evals_of_runs                                */
-    int64_t device_bytes;     /* memory charged against the budget                   */
+    int64_t device_bytes;     /* device bytes the request needs in an empty slot      */
     double  t_submit_ms;      /* server clock: milliseconds since adgpu_server_create */
     double  t_admit_ms;
     double  t_done_ms;
@@ -374,7 +388,7 @@ typedef struct adgpu_request_result {
 typedef struct adgpu_server_stats {
     int64_t submitted, completed, failed;
     int64_t queued, active, peak_active;
-    int64_t bytes_in_use, peak_bytes, memory_budget;
+    int64_t bytes_in_use, peak_bytes, memory_budget;   /* bytes the slots' buffers hold */
     int64_t slot_reallocations;   /* slot buffer growths after their first allocation */
     double  now_ms;
 } adgpu_server_stats;

@@ -96,3 +96,39 @@ def test_cli_device_autostop_equals_reference_host_autostop(tmp_path, lsmet):
     d1 = open(r1 + ".dlg", encoding="latin-1").read()
     d2 = open(r2 + ".dlg", encoding="latin-1").read()
     assert np.array_equal(_docked_energies(d1), _docked_energies(d2))
+
+
+def _dlg_runs(dlg_text):
+    """Per run of the dlg's DOCKED blocks: (run, inter, intra) as printed
+    (processresult.cpp ligand_calc_output, %+7.2f)."""
+    out = []
+    for block in dlg_text.split("DOCKED: USER    Run = ")[1:]:
+        run = int(block.split()[0])
+        inter = float(re.search(r"\(1\) Final Intermolecular Energy\s+=\s+([-+0-9.eE]+)", block).group(1))
+        intra = float(re.search(r"\(2\) Final Total Internal Energy\s+=\s+([-+0-9.eE]+)", block).group(1))
+        out.append((run, inter, intra))
+    return sorted(out)
+
+
+def test_cli_fp64_rescoring_of_each_runs_best_pose(tmp_path):
+    """The CLI prints the GPU's FP64 re-scoring of each run's best pose.  It
+    agrees with the dlg's energies of the same pose, which the reference's
+    CPU re-scoring computes (processresult.cpp:430-526): the intermolecular
+    energy within the dlg's 2-decimal rounding (same trilinear rule), the
+    intramolecular energy within 0.1 kcal/mol (the CPU uses distance tables;
+    on 400 docked poses of this ligand the analytic FP64 energy and the
+    tables differ by at most 0.036 kcal/mol, median 0.004)."""
+    res, out = _dock(tmp_path, "f64", "--nrun", "10", "--psize", "50", "--nev", "200000", "--seed", "3,5,7",
+                     "--heuristics", "0", "--autostop", "0", "--lsmet", "ad")
+    rows = [ln.split("|") for ln in out.splitlines() if re.match(r"^FP64\s+\d+\s+\|", ln)]
+    assert len(rows) == 10, out[-3000:]
+    f64 = {int(r[0].split()[1]): [float(x) for x in r[2:6]] for r in rows}
+    dlg = _dlg_runs(open(res + ".dlg", encoding="latin-1").read())
+    assert [r for r, _, _ in dlg] == list(range(1, 11))
+    for run, inter, intra in dlg:
+        e32, tot, i64, a64 = f64[run]
+        assert abs(tot - (i64 + a64)) <= 2e-4       # printed with 4 decimals
+        # the FP32 energy the search kept for the pose
+        assert abs(e32 - tot) <= 1e-4 * max(1.0, abs(tot)) + 2e-4, (run, e32, tot)
+        assert abs(i64 - inter) <= 0.0051 + 1e-4 * abs(inter), (run, i64, inter)
+        assert abs(a64 - intra) <= 0.1, (run, a64, intra)

@@ -21,7 +21,11 @@ from adgpu._abi import energy_params, search_params
 
 pytestmark = pytest.mark.gpu
 
-NRUN, POP, EVALS = 40, 150, 300000
+# 200 runs of 150 searches a generation (lsearch_rate 100 %) in two run
+# groups: 15000 searches per launch, more than the resident teams of every
+# case here (at most 8192 16-lane teams for half the device), so unstarted
+# searches remain when the first searches reach the hand-off point.
+NRUN, POP, EVALS = 200, 150, 300000
 
 
 def _ligand(n_atoms, n_tors, seed):
@@ -50,8 +54,9 @@ def _dock(ctx, mid, lid, sp, init, seeds, handoff):
     job.initpop()
     gens, total = job.run()
     st = job.download()
+    n = job.ls_handoffs()
     job.destroy()
-    return gens, total, st
+    return gens, total, st, n
 
 
 @pytest.mark.parametrize("atoms,tors,width", [(30, 6, 0), (20, 4, 16), (20, 4, 64), (70, 12, 32), (110, 20, 0),
@@ -67,9 +72,12 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
     gpu_ctx.set_team_width(width)
     try:
         ref = _dock(gpu_ctx, mid, lid, sp, init, seeds, 0)
+        assert ref[3] == 0
         for h in (1, 32):
-            print("atoms", atoms, "width", width, "hand-off", h, flush=True)
             got = _dock(gpu_ctx, mid, lid, sp, init, seeds, h)
+            print("atoms", atoms, "width", width, "hand-off", h, "searches handed off", got[3], flush=True)
+            # the hand-off took place (the bit-identity below is not vacuous)
+            assert got[3] > 0, (h, got[3])
             assert got[0] == ref[0] and got[1] == ref[1], (h, got[:2], ref[:2])
             for k in ref[2]:
                 assert np.array_equal(got[2][k], ref[2][k]), (h, k)

@@ -225,3 +225,69 @@ def test_one_collector_per_ticket(gpu_ctx, mix):
                 srv.result(t, timeout_ms=0)
         th.join(timeout=120)
         assert not th.is_alive() and got["r"]["status"] == 0
+
+
+def _device_free_bytes():
+    """hipMemGetInfo's free bytes of the current device (the HIP runtime the
+    library already loaded)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_admission_bounds_the_device_memory_slots_hold(gpu_ctx, mix):
+    """Slots keep their buffers for the next request, so the budget must bound
+    what idle slots hold too.  Six large requests (64 runs x 150, ADADELTA
+    with its hand-off state: ~18 MB each) alternate with small ones on eight
+    slots under a budget of 2.5 large requests.  Device memory measured with
+    hipMemGetInfo, while the requests run and after they finish, stays within
+    the budget (plus 10 % and 8 MB for the allocator's rounding); without the
+    bound, every slot that once ran a large request would keep its 18 MB.
+    Results are again the same as alone."""
+    import threading
+    import time
+    q = mix[0]
+    big_sp = search_params(num_runs=64, pop_size=150, max_evals=20000, ls="ad", spacing=0.375)
+    g = q["cg"].s
+    size_a = tuple((n - 1) * 0.375 for n in (g.size_x, g.size_y, g.size_z))
+    big_init = O.initial_population((9, 1), 64, 150, size_a, 0.375)
+    big_seeds = O.cmwc_uints((9, 1), 64 * 150 * 64)
+    reqs = []
+    for k in range(12):
+        if k % 2 == 0:
+            reqs.append(dict(q, sp=big_sp, init=big_init, seeds=big_seeds, autostop=None, k="big%d" % k))
+        else:
+            reqs.append(dict(mix[k % len(mix)], k="small%d" % k))
+    ref, _ = _serve(gpu_ctx, reqs, slots=1, one_at_a_time=True)      # also warms up the runtime
+    big = max(r["device_bytes"] for r in ref)
+    budget = int(2.5 * big)
+    with adgpu.Server(gpu_ctx, slots=8, memory_budget=budget) as srv:
+        time.sleep(0.05)
+        base = _device_free_bytes()
+        peak = [0]
+        stop = threading.Event()
+
+        def poll():
+            while not stop.is_set():
+                peak[0] = max(peak[0], base - _device_free_bytes())
+                time.sleep(0.001)
+        th = threading.Thread(target=poll)
+        th.start()
+        tickets = [srv.submit(r["mid"], r["T"], r["sp"], r["init"], r["seeds"], ep=r["ep"], autostop=r["autostop"],
+                              rescore_f64=True) for r in reqs]
+        served = [srv.result(t) for t in tickets]
+        stop.set()
+        th.join()
+        held = base - _device_free_bytes()
+        stats = srv.stats()
+    print("budget", budget, "big", big, "peak", peak[0], "held after", held, "stats", stats, flush=True)
+    slack = 0.10 * budget + (8 << 20)
+    assert stats["peak_bytes"] <= budget and stats["bytes_in_use"] <= budget
+    assert peak[0] <= budget + slack, (peak[0], budget)
+    assert held <= budget + slack, (held, budget)
+    assert 6 * big > budget + slack        # what slots would keep without the bound
+    for r, a, s in zip(reqs, ref, served):
+        assert s["status"] == 0, s["message"]
+        _same(a, s, r["k"])
