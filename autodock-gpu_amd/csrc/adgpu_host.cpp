@@ -60,6 +60,7 @@ struct adgpu_context {
     int scratch_n = 0;
     double* d_f64 = nullptr;   // FP64 re-scoring results [3n]
     int f64_n = 0;
+    float* noalloc = nullptr;  // this device's staging area of adgpu_eval_poses_noalloc
 };
 
 // Runs of a job are split into groups, each with its own stream, so that the
@@ -446,6 +447,10 @@ int adgpu_open(int device, adgpu_ctx* out) {
     }
     if (const char* y = std::getenv("ADGPU_LS_YIELD")) c->ls_yield = std::max(0, std::atoi(y));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
+    if (noalloc_scratch_setup(device, &c->noalloc) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        return ADGPU_ERR_HIP;
+    }
     if (const char* tw = std::getenv("ADGPU_TEAM_WIDTH")) {
         int v = std::atoi(tw);
         c->team_hint = (v == 16 || v == 32 || v == 64) ? v : 0;
@@ -587,6 +592,27 @@ int adgpu_eval_poses(adgpu_ctx c, int map_id, int lig_id, int mode, const float*
     if (grads && mode != ADGPU_EVAL_ENERGY)
         HIPCHK(c, hipMemcpyAsync(grads, c->d_grad, sizeof(float) * kGenes * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADGPU_OK;
+}
+
+int adgpu_eval_poses_noalloc(adgpu_ctx c, int map_id, int lig_id, int mode, const float* genotypes, int n,
+                             float* energies, float* grads) {
+    if (!c || !genotypes || n < 0) return fail(c, ADGPU_ERR_ARG, "null argument");
+    if (mode < ADGPU_EVAL_ENERGY || mode > ADGPU_EVAL_GRADIENT) return fail(c, ADGPU_ERR_ARG, "bad mode");
+    if (n > ADGPU_NOALLOC_MAX_POSES)
+        return fail(c, ADGPU_ERR_LIMIT, "adgpu_eval_poses_noalloc: n = " + std::to_string(n) +
+                                            " exceeds ADGPU_NOALLOC_MAX_POSES = " +
+                                            std::to_string(ADGPU_NOALLOC_MAX_POSES));
+    if (n == 0) return ADGPU_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto mi = c->maps.find(map_id);
+    auto li = c->ligands.find(lig_id);
+    if (mi == c->maps.end() || li == c->ligands.end()) return fail(c, ADGPU_ERR_ARG, "unknown map or ligand id");
+    if (li->second.max_map >= mi->second.dev.nmaps - 2)
+        return fail(c, ADGPU_ERR_ARG, "ligand atom_map index exceeds the grid's type maps");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, eval_poses_noalloc(launch_args(c, li->second, mi->second), c->noalloc, mode, genotypes, n, energies,
+                                 grads));
     return ADGPU_OK;
 }
 

@@ -140,6 +140,12 @@ hipError_t launch_ls_fire(const LaunchArgs& la, const SearchDev& sp, const JobBu
 hipError_t launch_autostop(hipStream_t s, const AutoStopParams& p, const float* ene, const int32_t* evals_runs,
                            AutoStopDev* st, float* bins, adgpu_autostop_record* log, int32_t* stop,
                            int64_t generation, int mode, adgpu_autostop_record* out);
+// Address of the staging area of adgpu_eval_poses_noalloc on `device` (which
+// must be current), looked up once per device (k_eval.hip).
+hipError_t noalloc_scratch_setup(int device, float** out);
+// adgpu_eval_poses through that staging area; n <= ADGPU_NOALLOC_MAX_POSES.
+hipError_t eval_poses_noalloc(const LaunchArgs& la, float* scratch_dev, int mode, const float* genotypes, int n,
+                              float* energies, float* grads);
 // Wavefront VALU instructions per second of a kernel of independent FP32
 // FMA chains over `blocks` workgroups of 256 threads (k_calib.hip); scratch
 // holds blocks * 256 floats.

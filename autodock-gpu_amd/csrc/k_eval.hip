@@ -1,6 +1,7 @@
 // k_eval.hip -- batched pose evaluation, the per-pose parity entry point
 // (gpu_calc_energy calcenergy.cl:132-554, gpu_calc_energrad
 // calcMergedEneGra.cl:46-936, gpu_calc_gradient calcgradient.cl:109-1003).
+#include <mutex>
 #include <type_traits>
 
 #include "adgpu_device.h"
@@ -32,6 +33,53 @@ __global__ ADGPU_GROUP_KERNEL void k_eval_poses(Launch a) {
 #pragma unroll
         for (int j = 0; j < GPL; ++j) a.out_g[(size_t)id * kGenes + tl + TW * j] = grad[j];
     }
+}
+
+// Staging area of adgpu_eval_poses_noalloc: genotypes [max][64], gradients
+// [max][64], energies [max].  Each device holds its own copy for the life of
+// the process; the host reaches it only through the address that
+// hipGetSymbolAddress returns on that device.
+static __device__ __align__(256) float scratch[(size_t)ADGPU_NOALLOC_MAX_POSES * (2 * kGenes + 1)];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_setup(int device, float** out) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // the caller has made `device` current
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// Host buffers in and out through the device's staging area: no allocation.
+hipError_t eval_poses_noalloc(const LaunchArgs& la, float* scratch_dev, int mode, const float* genotypes, int n,
+                              float* energies, float* grads) {
+    if (n <= 0) return hipSuccess;
+    if (n > ADGPU_NOALLOC_MAX_POSES || !scratch_dev) return hipErrorInvalidValue;
+    const size_t cap = (size_t)ADGPU_NOALLOC_MAX_POSES;
+    float* d_geno = scratch_dev;
+    float* d_grad = scratch_dev + cap * kGenes;
+    float* d_ene = scratch_dev + 2 * cap * kGenes;
+    hipError_t e = hipMemcpyAsync(d_geno, genotypes, sizeof(float) * kGenes * (size_t)n, hipMemcpyHostToDevice,
+                                  la.stream);
+    if (e == hipSuccess) e = launch_eval_poses(la, mode, d_geno, n, d_ene, d_grad);
+    if (e == hipSuccess && energies && mode != ADGPU_EVAL_GRADIENT)
+        e = hipMemcpyAsync(energies, d_ene, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, la.stream);
+    if (e == hipSuccess && grads && mode != ADGPU_EVAL_ENERGY)
+        e = hipMemcpyAsync(grads, d_grad, sizeof(float) * kGenes * (size_t)n, hipMemcpyDeviceToHost, la.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(la.stream);
+    return e;
 }
 
 size_t lds_bytes(const LigandDev& Lh, int team_hint, int* team_width) {

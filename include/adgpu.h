@@ -211,6 +211,19 @@ int  adgpu_ligand_release(adgpu_ctx ctx, int ligand_id);
 int  adgpu_eval_poses(adgpu_ctx ctx, int map_id, int ligand_id, int mode,
                       const float* genotypes, int n, float* energies, float* grads);
 
+/* As adgpu_eval_poses, but allocates nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * genotypes, energies and gradients pass through a fixed per-device staging
+ * area of ADGPU_NOALLOC_MAX_POSES poses (4.2 MB, held by each device the
+ * library runs on until the process ends); n above that returns
+ * ADGPU_ERR_LIMIT.  Calls on one device share that area: calls that overlap
+ * on one device (from different contexts on it) race.  Calls on one context
+ * are serialized by the context. */
+#define ADGPU_NOALLOC_MAX_POSES  8192
+int  adgpu_eval_poses_noalloc(adgpu_ctx ctx, int map_id, int ligand_id, int mode,
+                              const float* genotypes, int n, float* energies, float* grads);
+
 /* FP64 energies of n host genotypes on the GPU: total, inter and intra per
  * pose (any output may be NULL).  Same terms as gpu_calc_energy
  * (device/calcenergy.cl:132-554) evaluated in double; the final re-scoring
