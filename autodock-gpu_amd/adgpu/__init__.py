@@ -129,6 +129,10 @@ class Context:
                                               e.ctypes.data_as(_fp), gr.ctypes.data_as(_fp)), "adgpu_eval_poses")
         return e, gr
 
+    def set_graph_block(self, generations: int):
+        """Generations per captured hipGraph for jobs started afterwards (0, 2, 4 or 8; 0 = one by one)."""
+        self._check(self.lib.adgpu_set_graph_block(self.h, int(generations)), "adgpu_set_graph_block")
+
     def measure_valu_rate(self):
         """Wavefront VALU instructions per second on independent FP32 FMA chains."""
         r = C.c_double()

@@ -530,24 +530,33 @@ DEV float gather_finish(const GridDev& G, float3 c, float q, const Gather<MF>& g
 template <bool GRAD>
 DEV float pair_energy(const LigScalars& ls, const PairA& pa, const PairB& pb, float ad, float rdist, float inv_sp,
                       float& dedr) {
-    // vdW / H-bond with smoothing, r < 8 A
+    // vdW / H-bond with smoothing, r < 8 A: with qm = sd^-m and qn = sd^-n,
+    // E = AC qm - BD qn and dE/dsd = -(m AC qm - n BD qn) / sd.  sd^-12 is one
+    // reciprocal of sd^12 (a few ulp; the 12th power of 1/sd would carry up to
+    // 12 ulp), and sd^-n = sd^-12 sd^(12-n)
     const float delta = pb.reqm - ad;
     const float sd = (fabsf(delta) >= ls.half_smooth) ? ad + copysignf(ls.half_smooth, delta) : pb.reqm;
     const uint32_t fl = pa.flags;
-    const float fm = (float)(fl & 0xffu), fn = (float)((fl >> 8) & 0xffu);
+    const uint32_t nexp = (fl >> 8) & 0xffu;
+    const float q = fast_rcp(sd);
     const float r2 = sd * sd, r4 = r2 * r2, r6 = r4 * r2;
-    float rm = fast_rcp(r6 * r6);                     // m = 12
-    float rmn = ((fl >> 8) & 0xffu) == 6u ? r6 : r2;  // r^(m-n), n = 6 or 10
+    float qm = fast_rcp(r6 * r6);                     // m = 12
+    float qn = qm * (nexp == 6u ? r6 : r2);           // n = 6 or 10
+    float fn = (float)nexp;
+    bool general = false;
     if (__builtin_expect(ls.any_general, 0)) {        // other exponents (modified pairs): rare,
         if (!(fl & (1u << 17))) {                     // kept off the common path
+            general = true;
             const float l2 = __builtin_amdgcn_logf(sd);
-            rm = __builtin_amdgcn_exp2f(-fm * l2);
-            rmn = __builtin_amdgcn_exp2f((fm - fn) * l2);
+            qm = __builtin_amdgcn_exp2f(-(float)(fl & 0xffu) * l2);
+            qn = __builtin_amdgcn_exp2f(-fn * l2);
         }
     }
-    const float evdw = (pa.ac - rmn * pa.bd) * rm;
-    // electrostatics and desolvation, r < 20.48 A, r clamped at elec_min_distance
-    const float d = ad < ls.elec_min ? ls.elec_min : ad;
+    const float am = pa.ac * qm, bn = pa.bd * qn;
+    const float evdw = am - bn;
+    // electrostatics and desolvation, r < 20.48 A, r clamped at elec_min_distance:
+    // 1/max(r, r_min) = min(1/r, 1/r_min)
+    const float d = fmaxf(ad, ls.elec_min);
     const float d2 = d * d;
     const float desolv = pb.dsol * (ls.coeff_desolv * (12.96f - 0.1063f * d2 * (1.0f - 0.001947f * d2)) *
                                     fast_rcp(12.96f + d2 * (0.4137f + d2 * (0.00357f + 0.000112f * d2))));
@@ -555,17 +564,18 @@ DEV float pair_energy(const LigScalars& ls, const PairA& pa, const PairB& pb, fl
     const float rds = fast_rcp(ds);
     const float rds2 = rds * rds;
     const float diel = 1.10859f * rds2 + 0.010358f;
-    const float rd = ad < ls.elec_min ? ls.inv_elec_min : rdist * inv_sp;
+    const float rd = fminf(rdist * inv_sp, ls.inv_elec_min);
     const float es = pb.cq * rd;
     const float eel = diel * es + desolv;
-    const bool cgg0 = (fl & (1u << 16)) != 0u, in8 = ad < 8.0f, in20 = ad < 20.48f;
-    float e = cgg0 ? 50.0f * ad : 0.0f;               // CG-G0 bond term, G = 50
+    const bool in8 = ad < 8.0f, in20 = ad < 20.48f;
+    float e = pb.g * ad;                              // CG-G0 bond term, G = 50 (0 for other pairs)
     e += in8 ? evdw : 0.0f;
     e += in20 ? eel : 0.0f;
     if (GRAD) {
-        const float dvdw = (fn * pa.bd * rmn - fm * pa.ac) * rm * fast_rcp(sd);
+        const float mam = general ? (float)(fl & 0xffu) * am : 12.0f * am;
+        const float dvdw = (fn * bn - mam) * q;
         const float del = -es * rd * diel - es * 2.21718f * rds2 * rds - 0.0771605f * d * desolv;
-        float g = cgg0 ? 50.0f : 0.0f;
+        float g = pb.g;
         g += in8 ? dvdw : 0.0f;
         g += in20 ? del : 0.0f;
         dedr = g;
@@ -619,8 +629,9 @@ DEV float evaluate(const EvalCtx<TW, NC>& X, const float (&gene)[64 / TW], float
         const PairB pb = sv.pb[p];
         const int i = (int)(pa.ij & 0xffffu), j = (int)(pa.ij >> 16);
         const float sx = tv.cx[i] - tv.cx[j], sy = tv.cy[i] - tv.cy[j], sz = tv.cz[i] - tv.cz[j];
-        const float dist = fast_sqrt(sx * sx + sy * sy + sz * sz);
-        const float rdist = fast_rcp(dist);
+        const float d2g = sx * sx + sy * sy + sz * sz;
+        const float rdist = fast_rsq(d2g);            // one transcendental for r and 1/r
+        const float dist = d2g * rdist;
         float dedr;
         e += pair_energy<GRAD>(X.ls, pa, pb, dist * spacing, rdist, G.inv_spacing, dedr);
         if (GRAD) {

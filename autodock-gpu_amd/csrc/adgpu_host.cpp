@@ -51,6 +51,7 @@ struct adgpu_context {
     std::mutex mu;
     std::atomic<int> team_hint{0};   // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
     std::atomic<int> ls_yield{32};   // ADADELTA hand-off point in iterations, 0 = off (ADGPU_LS_YIELD)
+    std::atomic<int> graph_block{0}; // generations per captured graph, 0 = launched one by one (ADGPU_GRAPH_BLOCK)
     std::mutex table_mu;
     std::vector<std::pair<int, int>> team_table;   // (max atoms, width), ascending; adgpu_set_team_table
     // scratch for adgpu_eval_poses
@@ -75,8 +76,15 @@ struct RunGroup {
     int parity0 = 0;                    // parity when the current adgpu_job_run started
     int64_t done = 0, enq = 0;          // generations completed / enqueued in this adgpu_job_run
     std::vector<int64_t> totals;        // evals of the group's runs after each completed generation
-    bool ls_used[kRing] = {};
+    bool ls_used[kRing] = {};           // the slot's LS launch was timed by ev_ls
+    bool gen_timed[kRing] = {};         // the slot's K4 launch was timed by ev_gen
     hipEvent_t ev_done[kRing] = {}, ev_gen[2][kRing] = {}, ev_ls[2][kRing] = {};
+    // graph-captured blocks of generations, by the ring slot they start at:
+    // the block's launches, events and eval-total copies are those of its
+    // slots, so a block replays unchanged whenever its first slot comes round
+    // again with the same parity and local search
+    hipGraphExec_t blk[kRing] = {};
+    int blk_parity[kRing] = {}, blk_method[kRing] = {}, blk_len[kRing] = {};
 };
 
 constexpr int kMaxGroups = 8;
@@ -102,6 +110,7 @@ struct adgpu_job {
     int want_groups = 1;                // run groups of a serving slot (fewer when a request has fewer runs)
     int sharers = 1;                    // requests that may run at once with this job (a server's slots)
     int team_hint = 0;                  // team width of the job's kernels, fixed when it starts
+    int graph_block = 0;                // generations per captured graph, fixed when it starts (0: none)
     LigandSet own{};
     size_t own_cap = 0;
     float* d_best = nullptr;            // per-run best genotypes for FP64 re-scoring
@@ -121,6 +130,7 @@ struct adgpu_job {
     int32_t* ls_ring = nullptr;
     size_t cap_yield = 0;
     int32_t* h_ring = nullptr;          // pinned [kRing][num_runs] eval totals per generation
+    int32_t* d_ring = nullptr;          // device [kRing][num_runs]: the same, written by K2 in captured blocks
     std::vector<RunGroup> groups;
     int64_t generation = 0;
     double kernel_ms[2] = {0.0, 0.0};
@@ -333,7 +343,7 @@ int build_ligand(adgpu_context* c, const adgpu_ligand* L, const adgpu_energy_par
         rb.cq = ep->coeff_elec * q1 * q2;                       // calcenergy.cl:527 order
         rb.dsol = (L->ds_s[t1] + ep->qasp * std::fabs(q1)) * L->ds_v[t2] +
                   (L->ds_s[t2] + ep->qasp * std::fabs(q2)) * L->ds_v[t1];   // calcenergy.cl:488-491
-        rb.pad = 0.f;
+        rb.g = cgg0 ? 50.0f : 0.0f;                             // calcenergy.cl:452-458, G = 50
     }
     std::memcpy(blob.data(), &hd, sizeof(hd));
     return ADGPU_OK;
@@ -446,6 +456,10 @@ int adgpu_open(int device, adgpu_ctx* out) {
         g_wait_event_flags = hipEventDisableTiming | (v == "block" ? hipEventBlockingSync : 0u);
     }
     if (const char* y = std::getenv("ADGPU_LS_YIELD")) c->ls_yield = std::max(0, std::atoi(y));
+    if (const char* b = std::getenv("ADGPU_GRAPH_BLOCK")) {
+        const int v = std::atoi(b);
+        c->graph_block = (v == 2 || v == 4 || v == 8) ? v : 0;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ADGPU_ERR_HIP;
     if (noalloc_scratch_setup(device, &c->noalloc) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -668,8 +682,17 @@ static bool make_group(RunGroup& g) {
     return ok;
 }
 
+// Drops a group's captured generation blocks (their kernels hold the job's
+// buffers and the group's slice, which a new search may change).
+static void drop_blocks(RunGroup& g) {
+    for (int r = 0; r < kRing; ++r)
+        if (g.blk[r]) { (void)hipGraphExecDestroy(g.blk[r]); g.blk[r] = nullptr; }
+}
+
 static void destroy_group(RunGroup& g) {
-    if (g.stream) { (void)hipStreamSynchronize(g.stream); (void)hipStreamDestroy(g.stream); }
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    drop_blocks(g);   // before the stream its blocks were launched on
+    if (g.stream) (void)hipStreamDestroy(g.stream);
     for (int r = 0; r < kRing; ++r) {
         if (g.ev_done[r]) (void)hipEventDestroy(g.ev_done[r]);
         for (int h = 0; h < 2; ++h) {
@@ -678,6 +701,7 @@ static void destroy_group(RunGroup& g) {
         }
     }
     g = RunGroup{};
+    (void)hipGetLastError();   // nothing of a torn-down group may surface at the next launch
 }
 
 // Sets the number of run groups (at most one per run) and splits the runs
@@ -735,11 +759,14 @@ static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire, bool
     if (nrun > j->cap_runs) {
         if (j->cap_runs) j->reallocs++;
         (void)hipFree(j->evals_runs);
+        (void)hipFree(j->d_ring);
         if (j->h_ring) (void)hipHostFree(j->h_ring);
         j->evals_runs = nullptr;
+        j->d_ring = nullptr;
         j->h_ring = nullptr;
         j->cap_runs = 0;
         bool ok = hipMalloc(&j->evals_runs, nrun * sizeof(int32_t)) == hipSuccess &&
+                  hipMalloc(&j->d_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess &&
                   hipHostMalloc(&j->h_ring, (size_t)kRing * nrun * sizeof(int32_t)) == hipSuccess;
         if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
         j->cap_runs = nrun;
@@ -788,7 +815,9 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
     j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
+    j->graph_block = c->graph_block.load();
     for (auto& g : j->groups) {
+        drop_blocks(g);
         g.parity = g.parity0 = 0;
         g.done = g.enq = 0;
         g.totals.clear();
@@ -883,6 +912,7 @@ int adgpu_job_destroy(adgpu_job_t j) {
     (void)hipFree(j->pop[0]); (void)hipFree(j->pop[1]);
     (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
     (void)hipFree(j->evals_new); (void)hipFree(j->evals_runs); (void)hipFree(j->prng);
+    (void)hipFree(j->d_ring);
     (void)hipFree(j->counters); (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
     (void)hipFree(j->ls_state); (void)hipFree(j->ls_ring);
     (void)hipFree(j->d_stop); (void)hipFree(j->d_as); (void)hipFree(j->d_as_bins);
@@ -1001,6 +1031,14 @@ static int check_handoff(adgpu_job_t j) {
     return ADGPU_OK;
 }
 
+int adgpu_set_graph_block(adgpu_ctx c, int generations) {
+    if (!c) return ADGPU_ERR_ARG;
+    if (generations != 0 && generations != 2 && generations != 4 && generations != 8)
+        return fail(c, ADGPU_ERR_ARG, "graph block must be 0, 2, 4 or 8 generations");
+    c->graph_block = generations;
+    return ADGPU_OK;
+}
+
 int adgpu_measure_valu_rate(adgpu_ctx c, double* wave_instr_per_s) {
     if (!c || !wave_instr_per_s) return fail(c, ADGPU_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
@@ -1067,30 +1105,104 @@ int adgpu_job_generation(adgpu_job_t j) {
 
 // Enqueues one generation of group k (K4, local search, K2, eval totals to
 // the pinned ring) on its stream; performdocking.cpp.Cuda:700-831.
-static int enqueue_generation(adgpu_job_t j, int k, const LigandSet* ls, const MapSet* ms) {
+// captured: the generation goes into a graph block (enqueue_block), which
+// holds only the kernels; K2 then leaves the totals in the device ring, and
+// the block's launch copies them and records the completion events on the
+// stream after the graph.  Launches inside blocks are not timed.
+static int enqueue_generation(adgpu_job_t j, int k, const LigandSet* ls, const MapSet* ms, bool captured = false) {
     adgpu_context* c = j->ctx;
     RunGroup& g = j->groups[k];
     const int slot = (int)(g.enq % kRing);
     const LaunchArgs la = launch_args(c, *ls, *ms, g.stream, (int)j->groups.size(), j->team_hint);
     const SearchDev sd = slice_sd(j, g.r0, g.r1);
-    const JobBuffers b = slice(j, g.r0, g.r1, g.parity, k);
+    JobBuffers b = slice(j, g.r0, g.r1, g.parity, k);
     const int m = ls_method_for(j, j->generation + g.enq);
-    HIPCHK(c, hipEventRecord(g.ev_gen[0][slot], g.stream));
+    if (captured) b.evals_copy = j->d_ring + (size_t)slot * j->sp.num_runs + g.r0;
+    if (!captured) HIPCHK(c, hipEventRecord(g.ev_gen[0][slot], g.stream));
     HIPCHK(c, launch_gen(la, sd, b));
-    HIPCHK(c, hipEventRecord(g.ev_gen[1][slot], g.stream));
-    g.ls_used[slot] = m != ADGPU_LS_NONE;
-    if (g.ls_used[slot]) {
-        HIPCHK(c, hipEventRecord(g.ev_ls[0][slot], g.stream));
+    if (!captured) HIPCHK(c, hipEventRecord(g.ev_gen[1][slot], g.stream));
+    g.ls_used[slot] = m != ADGPU_LS_NONE && !captured;   // timed launches only
+    g.gen_timed[slot] = !captured;
+    if (m != ADGPU_LS_NONE) {
+        if (!captured) HIPCHK(c, hipEventRecord(g.ev_ls[0][slot], g.stream));
         HIPCHK(c, launch_ls(la, sd, b, m));
-        HIPCHK(c, hipEventRecord(g.ev_ls[1][slot], g.stream));
+        if (!captured) HIPCHK(c, hipEventRecord(g.ev_ls[1][slot], g.stream));
     }
     HIPCHK(c, launch_sum_evals(g.stream, sd, b));
-    HIPCHK(c, hipMemcpyAsync(j->h_ring + (size_t)slot * j->sp.num_runs + g.r0, j->evals_runs + g.r0,
-                             (size_t)(g.r1 - g.r0) * sizeof(int32_t), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(c, hipEventRecord(g.ev_done[slot], g.stream));
+    if (!captured) {
+        HIPCHK(c, hipMemcpyAsync(j->h_ring + (size_t)slot * j->sp.num_runs + g.r0, j->evals_runs + g.r0,
+                                 (size_t)(g.r1 - g.r0) * sizeof(int32_t), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(c, hipEventRecord(g.ev_done[slot], g.stream));
+    }
     g.parity ^= 1;
     g.enq++;
     return ADGPU_OK;
+}
+
+// Enqueues the next B generations of group k (B divides kRing; the first is
+// at a ring slot that is a multiple of B) as one graph launch.  The block is
+// captured from enqueue_generation the first time its first slot comes round
+// with this parity and local search, and replayed afterwards: its kernels are
+// those of its slots (K2 leaves each generation's totals in the slot's row of
+// the device ring), so a replay runs exactly the kernels of the B calls of
+// enqueue_generation.  After the graph, on the same stream, one copy brings
+// the B rows of totals to the host and the B completion events are recorded:
+// the host learns of each generation in stream order, as without the graph.
+static int enqueue_block(adgpu_job_t j, int k, const LigandSet* ls, const MapSet* ms, int B) {
+    adgpu_context* c = j->ctx;
+    RunGroup& g = j->groups[k];
+    const int slot0 = (int)(g.enq % kRing);
+    const int m = ls_method_for(j, j->generation + g.enq);
+    if (g.blk[slot0] && (g.blk_parity[slot0] != g.parity || g.blk_method[slot0] != m || g.blk_len[slot0] != B)) {
+        (void)hipGraphExecDestroy(g.blk[slot0]);
+        g.blk[slot0] = nullptr;
+    }
+    if (!g.blk[slot0]) {
+        const int parity = g.parity;
+        const int64_t enq = g.enq;
+        HIPCHK(c, hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+        int rc = ADGPU_OK;
+        for (int i = 0; i < B && rc == ADGPU_OK; ++i) rc = enqueue_generation(j, k, ls, ms, true);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(g.stream, &graph);
+        g.parity = parity;   // the launch below runs the captured generations
+        g.enq = enq;
+        if (rc != ADGPU_OK || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc != ADGPU_OK ? rc : fail(c, ADGPU_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                                   hipGetErrorString(ec));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(c, ADGPU_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        g.blk[slot0] = exec;
+        g.blk_parity[slot0] = parity;
+        g.blk_method[slot0] = m;
+        g.blk_len[slot0] = B;
+    }
+    HIPCHK(c, hipGraphLaunch(g.blk[slot0], g.stream));
+    const size_t nrun = (size_t)j->sp.num_runs, w = (size_t)(g.r1 - g.r0) * sizeof(int32_t);
+    HIPCHK(c, hipMemcpy2DAsync(j->h_ring + (size_t)slot0 * nrun + g.r0, nrun * sizeof(int32_t),
+                               j->d_ring + (size_t)slot0 * nrun + g.r0, nrun * sizeof(int32_t), w, (size_t)B,
+                               hipMemcpyDeviceToHost, g.stream));
+    for (int i = 0; i < B; ++i) {
+        const int slot = (slot0 + i) % kRing;
+        HIPCHK(c, hipEventRecord(g.ev_done[slot], g.stream));
+        g.ls_used[slot] = false;   // not timed
+        g.gen_timed[slot] = false;
+        g.parity ^= 1;
+        g.enq++;
+    }
+    return ADGPU_OK;
+}
+
+// Milliseconds between two recorded events; false when they cannot be timed
+// (the failure is not left behind for the next launch's error check).
+static bool elapsed_ms(hipEvent_t a, hipEvent_t b, float* ms) {
+    if (hipEventElapsedTime(ms, a, b) == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;
 }
 
 // Waits for the oldest in-flight generation of group k and records it.
@@ -1103,16 +1215,14 @@ static int complete_generation(adgpu_job_t j, int k) {
     for (int r = g.r0; r < g.r1; ++r) t += j->h_ring[(size_t)slot * j->sp.num_runs + r];
     g.totals.push_back(t);
     float ms = 0.f, a = 0.f, b = 0.f;
-    if (hipEventElapsedTime(&ms, g.ev_gen[0][slot], g.ev_gen[1][slot]) == hipSuccess) {
+    if (g.gen_timed[slot] && elapsed_ms(g.ev_gen[0][slot], g.ev_gen[1][slot], &ms)) {
         j->kernel_ms[0] += ms; j->kernel_n[0]++;
-        if (hipEventElapsedTime(&a, j->ev_origin, g.ev_gen[0][slot]) == hipSuccess &&
-            hipEventElapsedTime(&b, j->ev_origin, g.ev_gen[1][slot]) == hipSuccess)
+        if (elapsed_ms(j->ev_origin, g.ev_gen[0][slot], &a) && elapsed_ms(j->ev_origin, g.ev_gen[1][slot], &b))
             j->spans[0].emplace_back(a, b);
     }
-    if (g.ls_used[slot] && hipEventElapsedTime(&ms, g.ev_ls[0][slot], g.ev_ls[1][slot]) == hipSuccess) {
+    if (g.ls_used[slot] && elapsed_ms(g.ev_ls[0][slot], g.ev_ls[1][slot], &ms)) {
         j->kernel_ms[1] += ms; j->kernel_n[1]++;
-        if (hipEventElapsedTime(&a, j->ev_origin, g.ev_ls[0][slot]) == hipSuccess &&
-            hipEventElapsedTime(&b, j->ev_origin, g.ev_ls[1][slot]) == hipSuccess)
+        if (elapsed_ms(j->ev_origin, g.ev_ls[0][slot], &a) && elapsed_ms(j->ev_origin, g.ev_ls[1][slot], &b))
             j->spans[1].emplace_back(a, b);
     }
     g.done++;
@@ -1170,9 +1280,13 @@ static int stopped_generation(adgpu_job_t j, int64_t* gen) {
 }
 
 // Most evals one run can add in one generation: K4 offspring plus every LS
-// slot at its iteration cap (two evaluations per Solis-Wets iteration).
+// slot at its iteration cap: two evaluations per Solis-Wets iteration, one per
+// ADADELTA, ADAM or FIRE iteration (ADADELTA may run Solis-Wets first).
 static int64_t max_evals_per_generation(const adgpu_job* j) {
-    return (int64_t)(j->sp.pop_size - 1) + (int64_t)j->sd.num_ls * 2 * std::max<int64_t>(1, j->sp.max_num_of_iters);
+    const bool sw = j->sp.ls_method == ADGPU_LS_SW ||
+                    (j->sp.ls_method == ADGPU_LS_AD && j->sp.initial_sw_generations > 0);
+    return (int64_t)(j->sp.pop_size - 1) +
+           (int64_t)j->sd.num_ls * (sw ? 2 : 1) * std::max<int64_t>(1, j->sp.max_num_of_iters);
 }
 
 int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens, int64_t* out_total) {
@@ -1248,7 +1362,15 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
             for (int k = 0; k < (int)j->groups.size(); ++k) {
                 RunGroup& g = j->groups[k];
                 while (g.enq - g.done < kRing && g.enq < next_test && surely_continues(g.enq)) {
-                    if ((rc = enqueue_generation(j, k, ls, ms))) return rc;
+                    // a whole captured block when every generation of it surely runs
+                    // with one local search
+                    const int B = j->graph_block;
+                    bool block = B > 0 && g.enq % B == 0 && g.enq - g.done + B <= kRing && g.enq + B <= next_test &&
+                                 surely_continues(g.enq + B - 1);
+                    for (int i = 1; block && i < B; ++i)
+                        block = ls_method_for(j, j->generation + g.enq + i) == ls_method_for(j, j->generation + g.enq);
+                    rc = block ? enqueue_block(j, k, ls, ms, B) : enqueue_generation(j, k, ls, ms);
+                    if (rc) return rc;
                     moved = true;
                 }
             }
@@ -1551,7 +1673,7 @@ struct SlotCaps {
 static size_t caps_bytes(const SlotCaps& k) {
     size_t b = k.ne * (2 * kGenes * sizeof(float) + 2 * sizeof(float) + sizeof(int32_t) +
                        ADGPU_NWI * sizeof(uint32_t));
-    b += (size_t)k.runs * sizeof(int32_t);
+    b += (size_t)k.runs * sizeof(int32_t) * (1 + kRing);
     b += k.ls * (sizeof(int32_t) + kGenes * sizeof(float));
     b += k.yield * (kAdStateFloats * sizeof(float) + sizeof(int32_t));
     b += k.own;
@@ -1615,7 +1737,7 @@ void slot_shrink(adgpu_job* j) {
     for (auto& g : j->groups) (void)hipStreamSynchronize(g.stream);
     (void)hipFree(j->pop[0]); (void)hipFree(j->pop[1]);
     (void)hipFree(j->ene[0]); (void)hipFree(j->ene[1]);
-    (void)hipFree(j->evals_new); (void)hipFree(j->prng); (void)hipFree(j->evals_runs);
+    (void)hipFree(j->evals_new); (void)hipFree(j->prng); (void)hipFree(j->evals_runs); (void)hipFree(j->d_ring);
     (void)hipFree(j->ls_ent); (void)hipFree(j->ls_grad);
     (void)hipFree(j->ls_state); (void)hipFree(j->ls_ring);
     (void)hipFree(j->own.dev);
@@ -1623,7 +1745,7 @@ void slot_shrink(adgpu_job* j) {
     (void)hipFree(j->d_best); (void)hipFree(j->d_f64);
     if (j->h_ring) (void)hipHostFree(j->h_ring);
     j->pop[0] = j->pop[1] = j->ene[0] = j->ene[1] = nullptr;
-    j->evals_new = nullptr; j->prng = nullptr; j->evals_runs = nullptr; j->h_ring = nullptr;
+    j->evals_new = nullptr; j->prng = nullptr; j->evals_runs = nullptr; j->h_ring = nullptr; j->d_ring = nullptr;
     j->ls_ent = nullptr; j->ls_grad = nullptr; j->ls_state = nullptr; j->ls_ring = nullptr;
     j->own.dev = nullptr;
     j->d_as = nullptr; j->d_as_out = nullptr; j->d_as_bins = nullptr; j->d_as_log = nullptr;

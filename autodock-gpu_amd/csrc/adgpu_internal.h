@@ -27,7 +27,7 @@ struct __attribute__((aligned(16))) PairB {
     float reqm;       // reqm_AB (optimum distance)
     float cq;         // coeff_elec * q1 * q2 (same operation order as the reference)
     float dsol;       // (S1 + qasp|q1|) V2 + (S2 + qasp|q2|) V1
-    float pad;
+    float g;          // CG-G0 bond constant G: 50 for a CG-G0 pair, else 0
 };
 
 // Everything the kernels need about one ligand, in one device allocation.
@@ -90,6 +90,8 @@ struct JobBuffers {
     float* ls_state;       // ADADELTA hand-off: saved search per LS slot [runs*num_ls*kAdStateFloats]
     int32_t* ls_ring;      // ADADELTA hand-off: per claim of the queue, the search handed off there [runs*num_ls]
     const int32_t* stop;   // set by a passed AutoStop test: generation kernels return at once
+    int32_t* evals_copy;   // when set, K2 also writes the new per-run totals here (a generation
+                           // inside a captured graph block: its ring slot of the device copy)
 };
 
 // The AutoStop class's members (host/inc/autostop.hpp:36-58) in device

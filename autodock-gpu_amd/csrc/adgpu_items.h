@@ -179,7 +179,8 @@ DEV bool ad_step(AdItem<TW>& s, float e, const float (&g)[64 / TW], int NG, int 
         if (better) s.best[j] = s.geno[j];
         s.sqg[j] *= RHO;
         s.sqg[j] += (1.0f - RHO) * g[j] * g[j];
-        const float delta = -1.0f * g[j] * sqrtf((s.sqd[j] + EPS) / (s.sqg[j] + EPS));
+        // native square root of a native quotient, as kernel_ad.cl:447 computes it
+        const float delta = -1.0f * g[j] * __builtin_amdgcn_sqrtf((s.sqd[j] + EPS) * __builtin_amdgcn_rcpf(s.sqg[j] + EPS));
         s.sqd[j] *= RHO;
         s.sqd[j] += (1.0f - RHO) * delta * delta;
         s.geno[j] += delta;

@@ -49,14 +49,16 @@ __global__ ADGPU_GROUP_KERNEL void k_gen(Launch a) {
 
 // K2: per-run eval totals (kernel2.cl:26-62); one wavefront per run.
 __global__ __launch_bounds__(64) void k_sum_evals(SearchDev S, JobBuffers B) {
-    if (JOB_STOPPED(B)) return;
     const int run = blockIdx.x;
     const int lane = lane_id();
-    int s = 0;
-    for (int e = lane; e < S.pop_size; e += 64) s += B.evals_new[run * S.pop_size + e];
+    if (!JOB_STOPPED(B)) {
+        int s = 0;
+        for (int e = lane; e < S.pop_size; e += 64) s += B.evals_new[run * S.pop_size + e];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) B.evals_runs[run] += s;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) B.evals_runs[run] += s;
+    }
+    if (lane == 0 && B.evals_copy != nullptr) B.evals_copy[run] = B.evals_runs[run];
 }
 
 // Clears the local-search work queue and, for ADADELTA hand-offs, its

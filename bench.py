@@ -79,6 +79,8 @@ def parse():
     ap.add_argument("--cpu-evals", type=int, default=1000000,
                     help="evaluations per run of the CPU baseline's docking (of the 2.5 M)")
     ap.add_argument("--team-width", type=int, default=0, choices=[0, 16, 32, 64])
+    ap.add_argument("--graph-block", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="generations per captured hipGraph (adgpu_set_graph_block; 0 = launched one by one)")
     # hardware queues of this process, set before the HIP runtime starts (HIP's
     # default is 4): with 8 the library runs the job's runs in four groups on
     # four streams, each on its own queue
@@ -485,6 +487,8 @@ def serve(a):
     os.environ.setdefault("ADGPU_HOST_WAIT", "block")
     import adgpu
     ctx = adgpu.Context(0)
+    if a.graph_block:
+        ctx.set_graph_block(a.graph_block)
     sp = adgpu.search_params(num_runs=a.serve_nrun, pop_size=a.pop, max_evals=a.serve_evals, ls="fire",
                              spacing=0.375)
     mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
@@ -561,6 +565,8 @@ def screen(a):
     os.environ.setdefault("ADGPU_HOST_WAIT", "block")
     import adgpu
     ctx = adgpu.Context(0)
+    if a.graph_block:
+        ctx.set_graph_block(a.graph_block)
     sp = adgpu.search_params(num_runs=50, pop_size=a.pop, max_evals=a.evals, ls="adam", spacing=0.375)
     mids32 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F32) for m in map_sets]
     mids16 = [ctx.maps_upload(m.packed(), m.size, m.num_map_atypes, m.spacing, adgpu.MAP_F16) for m in map_sets]
@@ -628,6 +634,8 @@ def main():
     ctx = adgpu.Context(0)
     if a.team_width:
         ctx.set_team_width(a.team_width)
+    if a.graph_block:
+        ctx.set_graph_block(a.graph_block)
     mid = ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing, fmt)
     lid = ctx.ligand_upload(tables, ep)
     sp = adgpu.search_params(num_runs=a.nrun, pop_size=a.pop, max_evals=a.evals, ls=a.ls, spacing=maps.spacing)
@@ -678,7 +686,9 @@ def main():
     avg_ls_ms = tot_ls_ms / n_launch
     eff_ls_ms = float(np.sum(ls_busy)) / n_launch
     bytes_per_launch = ls_evals * bytes_per_eval / n_launch
-    achieved = bytes_per_launch / (eff_ls_ms * 1e-3) / 1e9
+    # launches inside captured graph blocks are not timed (--graph-block): the
+    # per-launch figure then needs a run without blocks
+    achieved = bytes_per_launch / (eff_ls_ms * 1e-3) / 1e9 if a.graph_block == 0 and eff_ls_ms > 0 else None
     tkey = (a.maps, a.nrun, a.pop, a.ls)
     traffic = (TRAFFIC_BYTES_PER_LS_EVAL[tkey] * ls_evals / max(1, int(np.sum(ls_n)))
                if tkey in TRAFFIC_BYTES_PER_LS_EVAL else None)
@@ -704,19 +714,19 @@ def main():
                                % ({"ad": "ADADELTA", "sw": "Solis-Wets", "adam": "ADAM", "fire": "FIRE"}.get(a.ls, a.ls),
                                   a.maps.upper(), a.nrun, a.pop, a.evals),
                    "nrun": a.nrun, "pop_size": a.pop, "evals_per_run": a.evals, "ls": a.ls, "maps": a.maps,
-                   "hw_queues": a.hw_queues},
+                   "hw_queues": a.hw_queues, "graph_block": a.graph_block},
         "ligands_per_hour": 3600.0 / float(np.mean(lat_arr)),
         "latency_s": {"p50": float(np.percentile(lat_arr, 50)), "p99": float(np.percentile(lat_arr, 99))},
         "generations_per_ligand": float(np.mean(gens_list)),
         "kernels": {"ls_avg_ms": avg_ls_ms, "ls_busy_ms_per_launch": eff_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
                     "ls_launches": int(np.sum(ls_n)), "ls_busy_share_of_wall": float(np.sum(ls_busy)) / 1000.0 / t},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "duration": "LS busy time (union of launch intervals) / launches",
                      "bytes_per_eval": bytes_per_eval, "aggregate_achieved": aggregate,
                      "aggregate_frac": aggregate / HBM_PEAK_GBS},
     }
-    if tkey in LS_INSTR_PER_EVAL:
+    if tkey in LS_INSTR_PER_EVAL and achieved:
         # VALU issue rate of the LS kernel while it runs (LS busy time) against
         # the calibrated rate of full-rate FP32 VALU instructions
         valu_per_eval, lds_per_eval = LS_INSTR_PER_EVAL[tkey]

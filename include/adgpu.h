@@ -286,7 +286,8 @@ int  adgpu_job_best_f64(adgpu_job_t job, float* best_genotypes, float* best_ener
 int  adgpu_job_kernel_time(adgpu_job_t job, int kernel_class, double* avg_ms, int64_t* launches);
 /* Device time during which at least one launch of the class ran (the union
  * of the launch intervals of all run groups since the job's first run call),
- * and the number of launches.  No counterpart in the reference: measurement
+ * and the number of launches.  Launches inside graph blocks
+ * (adgpu_set_graph_block) are not timed and not counted here.  No counterpart in the reference: measurement
  * for callers that time the kernels of overlapping run groups. */
 int  adgpu_job_kernel_busy(adgpu_job_t job, int kernel_class, double* busy_ms, int64_t* launches);
 
@@ -301,6 +302,17 @@ int  adgpu_job_ls_handoffs(adgpu_job_t job, int64_t* count);
  * peak that a kernel's measured VALU instruction rate is reported against.
  * No counterpart in the reference: measurement. */
 int  adgpu_measure_valu_rate(adgpu_ctx ctx, double* wave_instr_per_s);
+
+/* Generations per captured graph (0, 2, 4 or 8; default 0, or
+ * ADGPU_GRAPH_BLOCK) for jobs started afterwards.  The reference launches
+ * every kernel of every generation from its host loop
+ * (host/src/performdocking.cpp.Cuda:685-831); with a block size B each run
+ * group captures B consecutive generations (K4, local search, K2 and the
+ * eval-total copy, performdocking.cpp.Cuda:700-831) into a hipGraph once and
+ * replays it whenever the stop rule has proven all B generations run, so the
+ * host issues one launch per B generations.  Results are bit-identical to
+ * launching the generations one by one. */
+int  adgpu_set_graph_block(adgpu_ctx ctx, int generations);
 
 /* ---- AutoStop on the device -------------------------------------------------
  * The reference tests its population every as_frequency generations on the
