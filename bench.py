@@ -13,7 +13,10 @@ Prints one JSON line: evals/s (value), ligands/hour, p50/p99 per-ligand
 latency, the roofline of the dominant kernel (the local-search launch) and a
 CPU baseline: the CPU-runnable configuration (Solis-Wets, nrun 20, pop 150) run
 by the oracle's LGA on the host cores with a cut evaluation budget, timed to
-completion.
+completion.  "serving" adds the latency half of the metric under load: a
+short run of the serving workload below (96 ligands: saturation throughput,
+p50/p99 latency of Poisson arrivals at 70 % of it, and the plain one-at-a-time
+FP32 form on the first 8 ligands); --no-serving leaves it out.
 
 --workload poses evaluates 1e5 seeded random poses of the same ligand (energy
 and gradient) and compares each FP32 energy with the FP64 re-scoring of the
@@ -76,6 +79,10 @@ def parse():
     ap.add_argument("--ls", default="ad")
     ap.add_argument("--maps", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="lga: leave out the short serving-loop measurement (a child process running "
+                         "--workload serve on --serving-ligands ligands) attached to the line")
+    ap.add_argument("--serving-ligands", type=int, default=96)
     ap.add_argument("--cpu-evals", type=int, default=1000000,
                     help="evaluations per run of the CPU baseline's docking (of the 2.5 M)")
     ap.add_argument("--team-width", type=int, default=0, choices=[0, 16, 32, 64])
@@ -739,7 +746,34 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(maps, tables, ep, a.cpu_evals)
     ctx.close()
+    if not a.no_serving:
+        out["serving"] = serving_summary(a)
     print(json.dumps(out), flush=True)
+
+
+def serving_summary(a):
+    """The per-ligand latency part of the metric under load: the serving loop
+    (--workload serve) on a.serving_ligands uneven ligands, run in a child
+    process (its host threads wait for the device by blocking, which must be
+    set before the HIP runtime starts): saturation throughput, p50/p99 latency
+    of Poisson arrivals at 70 % of it, and the plain one-at-a-time FP32 form on
+    the first ligands."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "serve", "--serve-ligands",
+           str(a.serving_ligands), "--serve-phases", "saturation,poisson,plain", "--serve-plain", "8",
+           "--hw-queues", str(a.hw_queues)]
+    t0 = time.perf_counter()
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError("serving measurement failed: %s" % p.stderr[-2000:])
+    d = json.loads(p.stdout.strip().splitlines()[-1])
+    return {"workload": d["config"]["workload"], "slots": d["config"]["slots"],
+            "ligands_per_hour": d["value"], "evals_per_s": d["saturation"]["evals_per_s"],
+            "poisson": d.get("poisson"),
+            "plain_f32_ligands_per_hour": d.get("plain_f32_one_at_a_time", {}).get("ligands_per_hour"),
+            "speedup_vs_one_at_a_time": d.get("speedup_vs_one_at_a_time"),
+            "median_best_energy_f64": d.get("median_best_energy_f64"),
+            "seconds": time.perf_counter() - t0}
 
 
 if __name__ == "__main__":
