@@ -74,6 +74,14 @@ class Context:
         bit-identical either way; applies to jobs created afterwards."""
         self._check(self.lib.adgpu_set_ls_handoff(self.h, int(iterations)), "adgpu_set_ls_handoff")
 
+    def set_ls_regroup(self, iterations: int):
+        """Once a launch's ADADELTA searches have all started, a search that is
+        the last work of its wavefront moves to a team left idle beside a busy
+        one, trying every this many iterations (0: never; -1: every 8 in jobs
+        of four or more run groups, the default).  Bit-identical results;
+        applies to jobs created afterwards."""
+        self._check(self.lib.adgpu_set_ls_regroup(self.h, int(iterations)), "adgpu_set_ls_regroup")
+
     def set_team_table(self, table):
         """Per-size team widths [(max_atoms, width), ...] ascending; [] clears."""
         a = np.array([int(m) for m, _ in table] or [0], np.int32)
@@ -245,6 +253,13 @@ class Job:
         n = C.c_int64()
         self._c(self.lib.adgpu_job_ls_handoffs(self.h, C.byref(n)), "adgpu_job_ls_handoffs")
         return n.value
+
+    def ls_stats(self):
+        """{"handoffs", "regroups", "wave_steps"} since the job started
+        (adgpu_job_ls_stats)."""
+        a = (C.c_int64 * 4)()
+        self._c(self.lib.adgpu_job_ls_stats(self.h, a), "adgpu_job_ls_stats")
+        return {"handoffs": a[0], "regroups": a[1], "wave_steps": a[2]}
 
     def set_autostop(self, enable=True, stopstd=0.15, as_frequency=5, min_as_evals=0):
         """AutoStop tested on the device every as_frequency generations."""

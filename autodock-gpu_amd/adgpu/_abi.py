@@ -215,6 +215,8 @@ def load_library(path: str | None = None):
     lib.adgpu_ligand_release.argtypes = [vp, C.c_int]
     lib.adgpu_eval_poses.argtypes = [vp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, _fp, _fp]
     lib.adgpu_job_ls_handoffs.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.adgpu_job_ls_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.adgpu_set_ls_regroup.argtypes = [vp, C.c_int]
     lib.adgpu_measure_valu_rate.argtypes = [vp, C.POINTER(C.c_double)]
     lib.adgpu_set_graph_block.argtypes = [vp, C.c_int]
     lib.adgpu_eval_poses_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, _fp, _fp]
@@ -253,7 +255,7 @@ def load_library(path: str | None = None):
 EXPORTED = ["adgpu_open", "adgpu_close", "adgpu_last_error", "adgpu_version", "adgpu_set_team_width",
             "adgpu_set_ls_handoff",
             "adgpu_set_team_table", "adgpu_device_count", "adgpu_job_run_generations", "adgpu_rescore_f64", "adgpu_job_best_f64", "adgpu_maps_upload",
-            "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses", "adgpu_eval_poses_noalloc", "adgpu_job_ls_handoffs", "adgpu_measure_valu_rate", "adgpu_set_graph_block",
+            "adgpu_maps_release", "adgpu_ligand_upload", "adgpu_ligand_release", "adgpu_eval_poses", "adgpu_eval_poses_noalloc", "adgpu_job_ls_handoffs", "adgpu_job_ls_stats", "adgpu_set_ls_regroup", "adgpu_measure_valu_rate", "adgpu_set_graph_block",
             "adgpu_job_create", "adgpu_job_destroy", "adgpu_job_initpop", "adgpu_job_kernel_gen",
             "adgpu_job_kernel_ls", "adgpu_job_kernel_sum_and_swap", "adgpu_job_generation", "adgpu_job_run",
             "adgpu_job_download", "adgpu_job_upload", "adgpu_job_kernel_time", "adgpu_job_kernel_busy", "adgpu_job_set_autostop",

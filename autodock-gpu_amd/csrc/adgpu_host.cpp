@@ -51,6 +51,8 @@ struct adgpu_context {
     std::mutex mu;
     std::atomic<int> team_hint{0};   // 0 = automatic team width, else 16, 32 or 64 (ADGPU_TEAM_WIDTH)
     std::atomic<int> ls_yield{32};   // ADADELTA hand-off point in iterations, 0 = off (ADGPU_LS_YIELD)
+    std::atomic<int> ls_regroup{-1}; // ADADELTA regrouping once the queue drains: iterations, 0 = off,
+                                     // -1 = kRegroupAuto with 4 or more run groups (ADGPU_LS_REGROUP)
     std::atomic<int> graph_block{0}; // generations per captured graph, 0 = launched one by one (ADGPU_GRAPH_BLOCK)
     std::mutex table_mu;
     std::vector<std::pair<int, int>> team_table;   // (max atoms, width), ascending; adgpu_set_team_table
@@ -406,6 +408,13 @@ int adgpu_set_ls_handoff(adgpu_ctx c, int iterations) {
     return ADGPU_OK;
 }
 
+int adgpu_set_ls_regroup(adgpu_ctx c, int iterations) {
+    if (!c) return ADGPU_ERR_ARG;
+    if (iterations < -1) return fail(c, ADGPU_ERR_ARG, "regroup iterations must be >= -1");
+    c->ls_regroup = iterations;
+    return ADGPU_OK;
+}
+
 int adgpu_set_team_table(adgpu_ctx c, int n, const int32_t* max_atoms, const int32_t* widths) {
     if (!c || n < 0 || n > 16 || (n > 0 && (!max_atoms || !widths)))
         return fail(c, ADGPU_ERR_ARG, "team table: n must be 0..16 with both arrays");
@@ -456,6 +465,7 @@ int adgpu_open(int device, adgpu_ctx* out) {
         g_wait_event_flags = hipEventDisableTiming | (v == "block" ? hipEventBlockingSync : 0u);
     }
     if (const char* y = std::getenv("ADGPU_LS_YIELD")) c->ls_yield = std::max(0, std::atoi(y));
+    if (const char* y = std::getenv("ADGPU_LS_REGROUP")) c->ls_regroup = std::max(-1, std::atoi(y));
     if (const char* b = std::getenv("ADGPU_GRAPH_BLOCK")) {
         const int v = std::atoi(b);
         c->graph_block = (v == 2 || v == 4 || v == 8) ? v : 0;
@@ -790,12 +800,22 @@ static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire, bool
         j->ls_ring = nullptr;
         j->cap_yield = 0;
         bool ok = hipMalloc(&j->ls_state, nls_all * kAdStateFloats * sizeof(float)) == hipSuccess &&
-                  hipMalloc(&j->ls_ring, nls_all * sizeof(int32_t)) == hipSuccess;
+                  hipMalloc(&j->ls_ring, 3 * nls_all * sizeof(int32_t)) == hipSuccess;
         if (!ok) return fail(j->ctx, ADGPU_ERR_NOMEM, "job allocation failed");
         j->cap_yield = nls_all;
     }
     return ADGPU_OK;
 }
+
+// Regrouping interval of the automatic setting, used when a job runs in four
+// or more run groups: their launches overlap, so the lanes that regrouping
+// frees in one launch's last searches run another launch's work.  Measured on
+// the 30-atom ligand, ADADELTA nrun 100 x pop 150 (M evals/s, interval 8 / 32 /
+// off): 4 groups 736 / 726 / 711; 2 groups 469 / 685 / 676; 1 group 247 / 524 /
+// 575.  With fewer groups the last searches of a launch set its length, and
+// each hand-over attempt (a few device-scope memory round trips on one
+// counter that every lone search uses) lengthens them.
+constexpr int kRegroupAuto = 8;
 
 // True when the job's ADADELTA searches hand off between teams.
 static bool yield_wanted(const adgpu_context* c, const adgpu_search_params* sp) {
@@ -810,8 +830,11 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
     adgpu_context* c = j->ctx;
     j->sp = *sp;
     j->sd = to_dev(sp, lig.num_genes);
-    if (yield_wanted(c, sp) && j->cap_yield >= (size_t)sp->num_runs * std::max(1, j->sd.num_ls))
+    if (yield_wanted(c, sp) && j->cap_yield >= (size_t)sp->num_runs * std::max(1, j->sd.num_ls)) {
         j->sd.ls_yield = c->ls_yield.load();
+        const int rg = c->ls_regroup.load();
+        j->sd.ls_regroup = rg >= 0 ? rg : (j->groups.size() >= 4 ? kRegroupAuto : 0);
+    }
     j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
@@ -961,7 +984,7 @@ static JobBuffers slice(const adgpu_job* j, int r0, int r1, int parity, int coun
     b.ls_ent = j->ls_ent ? j->ls_ent + (size_t)r0 * nls : nullptr;
     b.ls_grad = j->ls_grad ? j->ls_grad + (size_t)r0 * nls * kGenes : nullptr;
     b.ls_state = j->ls_state ? j->ls_state + (size_t)r0 * nls * kAdStateFloats : nullptr;
-    b.ls_ring = j->ls_ring ? j->ls_ring + (size_t)r0 * nls : nullptr;
+    b.ls_ring = j->ls_ring ? j->ls_ring + 3 * (size_t)r0 * nls : nullptr;   // ring, then seats
     b.stop = j->d_stop;
     (void)r1;
     return b;
@@ -1052,6 +1075,22 @@ int adgpu_measure_valu_rate(adgpu_ctx c, double* wave_instr_per_s) {
     const hipError_t e = measure_valu_rate(c->stream, buf, blocks, 20000, wave_instr_per_s);
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(c, ADGPU_ERR_HIP, std::string("VALU rate: ") + hipGetErrorString(e));
+    return ADGPU_OK;
+}
+
+int adgpu_job_ls_stats(adgpu_job_t j, int64_t* stats) {
+    if (!j || !stats) return ADGPU_ERR_ARG;
+    HIPCHK(j->ctx, hipSetDevice(j->ctx->device));
+    for (auto& g : j->groups) HIPCHK(j->ctx, hipStreamSynchronize(g.stream));
+    int32_t q[16 * kMaxGroups];
+    HIPCHK(j->ctx, hipMemcpy(q, j->counters, sizeof(q), hipMemcpyDeviceToHost));
+    int64_t h = 0, r = 0, w = 0;
+    for (int k = 0; k < kMaxGroups; ++k) {
+        h += q[16 * k + 4];
+        r += q[16 * k + 7];
+        w += (uint32_t)q[16 * k + 8];
+    }
+    stats[0] = h; stats[1] = r; stats[2] = w; stats[3] = 0;
     return ADGPU_OK;
 }
 
@@ -1675,7 +1714,7 @@ static size_t caps_bytes(const SlotCaps& k) {
                        ADGPU_NWI * sizeof(uint32_t));
     b += (size_t)k.runs * sizeof(int32_t) * (1 + kRing);
     b += k.ls * (sizeof(int32_t) + kGenes * sizeof(float));
-    b += k.yield * (kAdStateFloats * sizeof(float) + sizeof(int32_t));
+    b += k.yield * (kAdStateFloats * sizeof(float) + 3 * sizeof(int32_t));
     b += k.own;
     if (k.as) b += sizeof(AutoStopDev) + sizeof(adgpu_autostop_record);
     b += k.as_bins * sizeof(float) + (size_t)k.as_log * sizeof(adgpu_autostop_record);

@@ -66,6 +66,8 @@ struct SearchDev {
     int32_t cons_limit, max_iters, ls_method;
     float adam_beta1, adam_beta2, adam_epsilon;
     int32_t ls_yield;   // ADADELTA: iterations after which a search may hand off (0: never)
+    int32_t ls_regroup; // ADADELTA with hand-off: iterations a team runs a search before the search may
+                        // move to a seat in another wavefront once the queue is drained (0: never)
 };
 
 // State of an ADADELTA search handed off between teams: genes, best genes,
@@ -84,11 +86,13 @@ struct JobBuffers {
     uint32_t* prng;
     int32_t* ls_counter;   // work queue of the persistent local-search kernels: [0] next unstarted
                            // search; ADADELTA hand-off: [2] next ring ticket, [3] fault flag
-                           // (sticky over a run)
+                           // (sticky over a run), [5] next seat ticket, [6] next seat claim;
+                           // per job: [4] hand-offs, [7] regroups, [8] wavefront evaluation steps
     int32_t* ls_ent;       // FIRE: entity chosen per LS slot [runs*num_ls]
     float* ls_grad;        // FIRE: initial gradient per LS slot [runs*num_ls*64]
     float* ls_state;       // ADADELTA hand-off: saved search per LS slot [runs*num_ls*kAdStateFloats]
-    int32_t* ls_ring;      // ADADELTA hand-off: per claim of the queue, the search handed off there [runs*num_ls]
+    int32_t* ls_ring;      // ADADELTA hand-off: per claim of the queue, the search handed off there
+                           // [runs*num_ls], then the regrouping seats [2*runs*num_ls]
     const int32_t* stop;   // set by a passed AutoStop test: generation kernels return at once
     int32_t* evals_copy;   // when set, K2 also writes the new per-run totals here (a generation
                            // inside a captured graph block: its ring slot of the device copy)

@@ -62,13 +62,14 @@ __global__ __launch_bounds__(64) void k_sum_evals(SearchDev S, JobBuffers B) {
 }
 
 // Clears the local-search work queue and, for ADADELTA hand-offs, its
-// counters and the n entries of the hand-off ring (a kernel rather than a
-// memset, so the launch sequence can be captured into a graph).  The fault
-// flag counter[3] stays: it is cleared when a search starts.
+// counters, the n entries of the hand-off ring and the 2n regrouping seats
+// that follow it (a kernel rather than a memset, so the launch sequence can be
+// captured into a graph).  The fault flag counter[3] and the job's tallies
+// (counter[4], [7], [8]) stay: they are cleared when a search starts.
 __global__ __launch_bounds__(256) void k_reset_queue(int32_t* counter, int32_t* ring, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 3) counter[i] = 0;
-    if (ring != nullptr && i < n) ring[i] = -1;
+    if (i < 3 || i == 5 || i == 6) counter[i] = 0;
+    if (ring != nullptr && i < 3 * n) ring[i] = -1;
 }
 
 hipError_t launch_initpop(const LaunchArgs& la, const SearchDev& sp, const JobBuffers& b) {
@@ -112,7 +113,7 @@ hipError_t launch_ls(const LaunchArgs& la, const SearchDev& sp, const JobBuffers
     if (sp.num_runs * sp.num_ls <= 0 || method == ADGPU_LS_NONE) return hipSuccess;
     const int n = sp.num_runs * sp.num_ls;
     int32_t* ring = method == ADGPU_LS_AD && sp.ls_yield > 0 ? b.ls_ring : nullptr;
-    hipLaunchKernelGGL(k_reset_queue, dim3(ring != nullptr ? (n + 255) / 256 : 1), dim3(256), 0, la.stream,
+    hipLaunchKernelGGL(k_reset_queue, dim3(ring != nullptr ? (3 * n + 255) / 256 : 1), dim3(256), 0, la.stream,
                        b.ls_counter, ring, n);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;

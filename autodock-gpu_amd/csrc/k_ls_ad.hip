@@ -55,6 +55,35 @@ DEV int take_handoff(int32_t* Q, const int32_t* ring, int ntot, int& ticket, int
     return -1;
 }
 
+// Seat entries (regrouping): kRingUnset until a search is handed to the seat
+// (its LS slot) or the seat is given up.
+constexpr int kSeatCancelled = -3;
+
+// Team lane 0, a team with a search that no other team of its wavefront can
+// share: claims the oldest seat ticket no search has been handed to yet
+// (Q[6] < Q[5]) if it is below `cap`.  Returns the ticket, or -1 when none is
+// free or another team claimed it first.
+DEV int claim_seat(int32_t* Q, int cap) {
+    int t = __hip_atomic_load(&Q[6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int h = __hip_atomic_load(&Q[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t >= h || t >= cap) return -1;
+    // one attempt: many lone searches try at once, and retries would queue on
+    // the one counter
+    if (__hip_atomic_compare_exchange_strong(&Q[6], &t, t + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+        return t;
+    return -1;
+}
+
+// Team lane 0: moves seat entry *e from kRingUnset to v.  Returns the entry's
+// value afterwards (v on success; the search handed to it, or kSeatCancelled,
+// when another team got there first).
+DEV int seat_swap(int32_t* e, int v) {
+    int u = kRingUnset;
+    __hip_atomic_compare_exchange_strong(e, &u, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return u == kRingUnset ? v : u;
+}
+
 // Each team takes LS slots (run, entity) from the queue and runs one
 // ADADELTA search per slot (adgpu_items.h).
 //
@@ -71,6 +100,21 @@ DEV int take_handoff(int32_t* Q, const int32_t* ring, int ntot, int& ticket, int
 // claim precedes the first ticket, so a ticket waits only for an entry its
 // claimer is about to write, and waits by looking once per iteration, so its
 // wavefront partner keeps running.
+//
+// Regrouping (S.ls_regroup > 0, teams narrower than the wavefront): once the
+// queue and the ring are drained, a team whose wavefront still runs another
+// team's search, but has none of its own, takes a seat (a ticket Q[5] and a
+// seat entry) and looks at it once per iteration; a team whose search is the
+// only work left in its wavefront, after running it for ls_regroup iterations
+// there, saves it and hands it to the oldest free seat (a ticket Q[6] < Q[5])
+// if that seat is older than every seat its own wavefront holds: two
+// half-idle wavefronts become one busy and one finished, and the lanes the
+// idle teams spent on evaluations nobody used go to other launches.  A seated
+// team whose wavefront has no other search left gives its seat up; giving up
+// and handing over race on the seat entry (one compare-and-swap each), so a
+// search handed to a seat is taken exactly once, and a hand-over that loses
+// keeps the search where it is.  Like the ring hand-off it computes exactly
+// what the search would have computed on its first team.
 template <int TW, int NC, int MF, bool PL>
 __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
     if (JOB_STOPPED(a.B)) return;
@@ -81,8 +125,12 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
     int32_t* const Q = a.B.ls_counter;
     int32_t* const ring = a.B.ls_ring;
     const int yq = ring != nullptr && a.B.ls_state != nullptr ? S.ls_yield : 0;
+    const int rk = TW < 64 && yq > 0 ? S.ls_regroup : 0;   // regrouping: seats follow the ring
+    int32_t* const seats = rk > 0 ? ring + ntot : nullptr;
+    const int seat_cap = 2 * ntot;
     // While a team has no search, its AdItem is dead: cons_succ and cons_fail
-    // then hold lane 0's ring ticket (-1: none) and its poll count.
+    // then hold lane 0's ring ticket (-1: none) and its poll count, and once
+    // the ring is drained cons_fail holds the team's seat while it is seated.
     AdItem<TW> it;
 #pragma unroll
     for (int j = 0; j < GPL; ++j) { it.geno[j] = 0.f; it.best[j] = 0.f; it.sqg[j] = 0.f; it.sqd[j] = 0.f; }
@@ -92,6 +140,10 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
     bool active = false, done = false, need = true;
     bool fresh = true;               // unstarted slots may remain
     bool may_yield = false;          // the search in progress may still be handed off
+    bool seated = false;             // holds seat it.cons_fail, waiting for a search
+    bool moved = false;              // the search in progress came from a seat (it moves once)
+    int peek = kRingUnset;           // lane 0 of a seated team: its seat entry, read last iteration
+    uint32_t steps = 0;              // wavefront evaluation steps (tallied once per wavefront)
     for (;;) {
         if (need) {   // uniform within the team
             int c = -1;
@@ -120,6 +172,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
                         may_yield = false;
                         active = true;
                         need = false;
+                        moved = false;
                     } else {
                         if (tl == 0) __hip_atomic_store(&Q[3], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         it.cons_succ = -1;
@@ -140,6 +193,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
                 may_yield = yq > 0;
                 active = true;
                 need = false;
+                moved = false;
             } else {
                 active = false;
                 done = true;
@@ -150,13 +204,66 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
                 for (int j = 0; j < GPL; ++j) it.geno[j] = 0.f;
             }
         }
-        if (!TM::any_in_wave(!done)) break;
-        if (!TM::any_in_wave(active)) {   // every team of the wavefront waits for a ring entry
+        if (rk > 0) {   // regrouping (uniform over the wavefront from here to its end)
+            // another team of the wavefront has a search (teams are uniform: TW lanes each)
+            const bool others = __popcll(__ballot(active)) > (active ? TW : 0);
+            if (seated) {
+                // the entry as read before the last evaluation (a written entry does
+                // not change again); giving the seat up re-reads it by compare-and-swap
+                int v = peek;
+                if (tl == 0 && v == kRingUnset && !others) v = seat_swap(&seats[it.cons_fail], kSeatCancelled);
+                v = TM::bcast_i(v, 0);
+                if (v >= 0) {   // a search was handed to this seat
+                    seated = false;
+                    t = v;
+                    gi = ad_load<TW>(it, a.B.ls_state + (size_t)t * kAdStateFloats);
+                    const int run = t / nls;
+                    const bool valid = it.iters >= 1 && it.iters < S.max_iters &&
+                                       gi >= run * pop && gi < (run + 1) * pop;
+                    if (valid) {
+                        may_yield = false;
+                        active = true;
+                        moved = true;
+                    } else {
+                        if (tl == 0) __hip_atomic_store(&Q[3], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        it.cons_succ = -1;
+                        it.cons_fail = 0;
+                    }
+                } else if (v == kSeatCancelled) {
+                    seated = false;
+                    it.cons_fail = 0;
+                }
+            } else if (done && !active && others) {
+                int h = 0;
+                if (tl == 0) h = atomicAdd(&Q[5], 1);
+                h = TM::bcast_i(h, 0);
+                if (h < seat_cap) { seated = true; it.cons_fail = h; peek = kRingUnset; }
+            }
+        }
+        if (!TM::any_in_wave(!done || active || seated)) {
+            if (lane_id() == 0 && steps != 0u) atomicAdd(&Q[8], (int)steps);
+            break;
+        }
+        if (!TM::any_in_wave(active)) {   // every team of the wavefront waits for a ring entry or a seat
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
+        if (rk > 0 && seated && tl == 0)   // read now, used next iteration: the evaluation hides the latency
+            peek = __hip_atomic_load(&seats[it.cons_fail], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         float g[GPL];
+        ++steps;
         const float e = evaluate<TW, NC, MF, true>(X, it.geno, g);
+        // no other team of the wavefront has a search or may still get one from
+        // the queue or the ring; a search moves only to a seat older than every
+        // seat its wavefront holds, so two half-idle wavefronts merge toward the
+        // older seat instead of trading searches
+        const bool alone = rk > 0 && __popcll(__ballot(active || !done)) == ((active || !done) ? TW : 0);
+        int lim = seat_cap;
+        if (rk > 0) {
+            const int ms = seated ? it.cons_fail : seat_cap;
+#pragma unroll
+            for (int k = 0; k < 64 / TW; ++k) lim = min(lim, rdl_i(ms, k * TW));
+        }
         if (!active) continue;
         if (ad_step<TW>(it, e, g, NG, S.max_iters)) {
             ad_store<TW>(it, NG, a.B.pop_next + (size_t)gi * kGenes, a.B.ene_next + gi, a.B.evals_new + gi);
@@ -164,6 +271,37 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_ad(Launch a) {
             it.cons_fail = 0;
             active = false;
             need = true;
+        } else if (rk > 0 && alone && !moved && it.iters % rk == 0) {
+            // (alone: the wavefront's other teams drew no work from the queue or
+            // the ring, so both are drained)
+            // the wavefront's last search, every rk iterations: hand it to a seat
+            // in a busier wavefront (a search moved there runs at least one
+            // iteration before it may move again)
+            int tk = -1;
+            if (tl == 0) {
+                tk = claim_seat(Q, lim);
+                // a seat given up already: no record to save (the ticket is spent)
+                if (tk >= 0 && __hip_atomic_load(&seats[tk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kRingUnset)
+                    tk = -1;
+            }
+            tk = TM::bcast_i(tk, 0);
+            if (tk >= 0) {
+                ad_save<TW>(it, gi, a.B.ls_state + (size_t)t * kAdStateFloats);
+                __builtin_amdgcn_s_waitcnt(kWaitStores);   // the record is written before the seat entry
+                int r = 0;
+                if (tl == 0) {
+                    r = seat_swap(&seats[tk], t);
+                    if (r == t) atomicAdd(&Q[7], 1);   // regroups of the job (not cleared per launch)
+                }
+                r = TM::bcast_i(r, 0);
+                if (r == t) {
+                    // drained: the next acquisition finds the queue and the ring used up
+                    it.cons_succ = -1;
+                    it.cons_fail = 0;
+                    active = false;
+                    need = true;
+                }
+            }
         } else if (may_yield && it.iters == yq) {
             may_yield = false;
             int c = -1;

@@ -650,7 +650,9 @@ def main():
     bytes_per_eval = n_inter * 3 * 8 * {"f32": 4, "f16": 2, "bf16": 2}[a.maps]
 
     lat, evals_list, gens_list, ls_ms, ls_n, gen_ms, ls_busy = [], [], [], [], [], [], []
-    ls_evals_list = []
+    ls_evals_list, ls_stats_list = [], []
+    # lanes per team: the library runs ligands of at most 32 atoms in 32-lane teams
+    tw_used = a.team_width if a.team_width else 32
     for step in range(a.warmup + a.steps):
         init, seeds = initial_state(1000 + step, a.nrun, a.pop, maps)
         job = ctx.job(mid, lid, sp, init, seeds)      # allocations outside the timed region
@@ -661,6 +663,7 @@ def main():
         ms_ls, n_ls = job.kernel_time(1)
         busy_ls, _ = job.kernel_busy(1)
         ms_gen, n_gen = job.kernel_time(0)
+        lss = job.ls_stats()
         st = job.download()
         job.destroy()
         if step >= a.warmup:
@@ -673,6 +676,7 @@ def main():
             gen_ms.append(ms_gen)
             # LS evaluations: total minus K1 (nrun*pop) and K4 (nrun*(pop-1) per generation)
             ls_evals_list.append(total - a.nrun * a.pop - gens * a.nrun * (a.pop - 1))
+            ls_stats_list.append(lss)
         print("step %d: %.3f s, %d generations, %d evals, best %.3f" %
               (step, dt, gens, total, float(st["ene"].min())), file=sys.stderr, flush=True)
 
@@ -726,7 +730,13 @@ def main():
         "latency_s": {"p50": float(np.percentile(lat_arr, 50)), "p99": float(np.percentile(lat_arr, 99))},
         "generations_per_ligand": float(np.mean(gens_list)),
         "kernels": {"ls_avg_ms": avg_ls_ms, "ls_busy_ms_per_launch": eff_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
-                    "ls_launches": int(np.sum(ls_n)), "ls_busy_share_of_wall": float(np.sum(ls_busy)) / 1000.0 / t},
+                    "ls_launches": int(np.sum(ls_n)), "ls_busy_share_of_wall": float(np.sum(ls_busy)) / 1000.0 / t,
+                    # ADADELTA: searches handed off and regrouped per docking, and the share
+                    # of team evaluation steps idle teams spent beside busy ones
+                    "ls_handoffs": float(np.mean([x["handoffs"] for x in ls_stats_list])),
+                    "ls_regroups": float(np.mean([x["regroups"] for x in ls_stats_list])),
+                    "idle_team_share": (1.0 - ls_evals / (sum(x["wave_steps"] for x in ls_stats_list) * 64.0 / tw_used)
+                                        if a.ls == "ad" and sum(x["wave_steps"] for x in ls_stats_list) else None)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "duration": "LS busy time (union of launch intervals) / launches",

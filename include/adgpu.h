@@ -190,6 +190,18 @@ int  adgpu_set_team_table(adgpu_ctx ctx, int n, const int32_t* max_atoms, const 
  * (cuda/kernel_ad.cu:60-120: one block per run and LS entity); this has no
  * counterpart there and changes only when each search runs. */
 int  adgpu_set_ls_handoff(adgpu_ctx ctx, int iterations);
+/* ADADELTA regrouping, with the hand-off on and teams narrower than a
+ * wavefront: once a launch's searches have all started, a team left without a
+ * search beside a team that still runs one takes a seat, and a search that is
+ * the last work of its wavefront moves (once) to a free seat, trying every
+ * `iterations` iterations, so wavefronts that would run one team's search
+ * beside idle lanes merge and the freed lanes run other launches' work.
+ * Bit-identical results, like the hand-off; 0 turns it off; -1 (the default,
+ * also environment variable ADGPU_LS_REGROUP at adgpu_open) regroups every 8
+ * iterations in jobs of four or more run groups, whose launches overlap, and
+ * not otherwise.  Applies to jobs created afterwards.  No counterpart in the
+ * reference (one search per block, cuda/kernel_ad.cu:60-120). */
+int  adgpu_set_ls_regroup(adgpu_ctx ctx, int iterations);
 
 /* --- resident inputs ---------------------------------------------------------
  * Replace the per-job uploads in docking_with_gpu
@@ -296,6 +308,15 @@ int  adgpu_job_kernel_busy(adgpu_job_t job, int kernel_class, double* busy_ms, i
  * every search runs to its end in its own block (cuda/kernel_ad.cu:60-120):
  * measurement, for callers and tests that check the hand-off took place. */
 int  adgpu_job_ls_handoffs(adgpu_job_t job, int64_t* count);
+/* Local-search tallies since the job started: stats[0] hand-offs (as
+ * adgpu_job_ls_handoffs), stats[1] regroups (searches moved to a seat,
+ * adgpu_set_ls_regroup), stats[2] wavefront evaluation steps of the ADADELTA
+ * launches (each counts one evaluation for every team of the wavefront, idle
+ * teams included, modulo 2^32 per run group), stats[3] 0.  With the job's
+ * local-search evaluations E and teams of TW lanes, 1 - E / (stats[2] * 64 / TW)
+ * is the share of team evaluation steps that idle teams spent.  No counterpart
+ * in the reference: measurement. */
+int  adgpu_job_ls_stats(adgpu_job_t job, int64_t stats[4]);
 
 /* Wavefront VALU instructions per second the device issues on independent
  * full-rate FP32 FMA chains (v_fma_f32, 32 wavefronts per compute unit): the

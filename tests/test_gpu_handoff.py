@@ -11,6 +11,12 @@ and four atom chunks per lane, where the kernel variants (and their register
 budgets) differ.  The reference runs each search to its end in its own block
 (cuda/kernel_ad.cu:60-120); its results do not depend on the order in which
 searches run, and neither do these.
+
+Regrouping (adgpu_set_ls_regroup) moves, once a launch's searches have all
+started, a search that is the last work of its wavefront to a team left idle
+beside a busy one; it goes through the same saved records, so the dockings
+are bit-identical with regrouping off, trying every iteration, or every 8,
+and the tests check that searches did move.
 """
 import numpy as np
 import pytest
@@ -48,15 +54,18 @@ def _initial(nrun, pop, maps, seed):
     return init, seeds
 
 
-def _dock(ctx, mid, lid, sp, init, seeds, handoff):
+def _dock(ctx, mid, lid, sp, init, seeds, handoff, regroup=8):
     ctx.set_ls_handoff(handoff)
+    ctx.set_ls_regroup(regroup)
     job = ctx.job(mid, lid, sp, init, seeds)
     job.initpop()
     gens, total = job.run()
     st = job.download()
     n = job.ls_handoffs()
+    stats = job.ls_stats()
     job.destroy()
-    return gens, total, st, n
+    assert stats["handoffs"] == n
+    return gens, total, st, n, stats
 
 
 @pytest.mark.parametrize("atoms,tors,width", [(30, 6, 0), (20, 4, 16), (20, 4, 64), (70, 12, 32), (110, 20, 0),
@@ -72,17 +81,26 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
     gpu_ctx.set_team_width(width)
     try:
         ref = _dock(gpu_ctx, mid, lid, sp, init, seeds, 0)
-        assert ref[3] == 0
-        for h in (1, 32):
-            got = _dock(gpu_ctx, mid, lid, sp, init, seeds, h)
-            print("atoms", atoms, "width", width, "hand-off", h, "searches handed off", got[3], flush=True)
+        assert ref[3] == 0 and ref[4]["regroups"] == 0
+        for h, rg in ((1, 8), (32, 0), (32, 8), (32, 1)):
+            got = _dock(gpu_ctx, mid, lid, sp, init, seeds, h, rg)
+            st = got[4]
+            print("atoms", atoms, "width", width, "hand-off", h, "regroup", rg, "searches handed off", got[3],
+                  "regrouped", st["regroups"], "wavefront steps", st["wave_steps"], flush=True)
             # the hand-off took place (the bit-identity below is not vacuous)
             assert got[3] > 0, (h, got[3])
-            assert got[0] == ref[0] and got[1] == ref[1], (h, got[:2], ref[:2])
+            # searches regrouped, except with one team per wavefront (nothing to merge)
+            tw = width if width else (64 if atoms > 128 else 32)
+            if rg > 0 and tw < 64:
+                assert st["regroups"] > 0, st
+            else:
+                assert st["regroups"] == 0, st
+            assert got[0] == ref[0] and got[1] == ref[1], (h, rg, got[:2], ref[:2])
             for k in ref[2]:
-                assert np.array_equal(got[2][k], ref[2][k]), (h, k)
+                assert np.array_equal(got[2][k], ref[2][k]), (h, rg, k)
     finally:
         gpu_ctx.set_ls_handoff(32)
+        gpu_ctx.set_ls_regroup(-1)
         gpu_ctx.set_team_width(0)
         gpu_ctx.ligand_release(lid)
         gpu_ctx.maps_release(mid)
@@ -91,3 +109,5 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
 def test_handoff_argument(gpu_ctx):
     with pytest.raises(adgpu.AdgpuError):
         gpu_ctx.set_ls_handoff(-1)
+    with pytest.raises(adgpu.AdgpuError):
+        gpu_ctx.set_ls_regroup(-2)
