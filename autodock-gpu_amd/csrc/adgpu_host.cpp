@@ -191,9 +191,13 @@ LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, 
 constexpr int64_t kWideTeamSearches = 20000;
 
 // A job's team width, fixed when it starts: the context's width or table,
-// else 32 lanes up to 32 atoms and 64 above 128; between, 64 lanes when the
+// else 32 lanes up to 32 atoms and 64 above 64; between, 64 lanes when the
 // searches in flight -- the job's own times the requests that may share the
 // device with it (a server's slots) -- are below kWideTeamSearches, else 32.
+// Above 64 atoms a 32-lane team carries three or four atom chunks per lane;
+// measured with 16 requests of nrun 20 served at once (48000 searches), M
+// evals/s at 72 / 90 / 110 atoms, 32 vs 64 lanes: ADADELTA 52 / 33 / 19 vs
+// 86 / 57 / 35, FIRE 135 / 99 / 71 vs 135 / 100 / 74.
 // Measured, one job alone: lognormal 8-90-atom mix, FIRE nrun 20 (3000
 // searches) 0.101 s per ligand (32) vs 0.081 s (64); ligand + 2 flexible
 // residues (54 atoms), ADADELTA nrun 100 (15000 searches) 198 vs 226 M
@@ -205,7 +209,7 @@ int job_team_width(adgpu_context* c, int num_atoms, int64_t searches, int sharer
     const int w = context_team_width(c, num_atoms);
     if (w) return w;
     if (num_atoms <= 32) return 32;
-    if (num_atoms > 128) return 64;
+    if (num_atoms > 64) return 64;
     return searches * std::max(1, sharers) < kWideTeamSearches ? 64 : 32;
 }
 

@@ -90,7 +90,8 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
             # the hand-off took place (the bit-identity below is not vacuous)
             assert got[3] > 0, (h, got[3])
             # searches regrouped, except with one team per wavefront (nothing to merge)
-            tw = width if width else (64 if atoms > 128 else 32)
+            # default widths of these jobs: 32 lanes at 30 atoms, 64 above 64 atoms
+            tw = width if width else (64 if atoms > 64 else 32)
             if rg > 0 and tw < 64:
                 assert st["regroups"] > 0, st
             else:
