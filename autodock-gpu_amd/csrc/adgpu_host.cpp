@@ -1191,6 +1191,18 @@ static int enqueue_generation(adgpu_job_t j, int k, const LigandSet* ls, const M
 // enqueue_generation.  After the graph, on the same stream, one copy brings
 // the B rows of totals to the host and the B completion events are recorded:
 // the host learns of each generation in stream order, as without the graph.
+// Generations a group must have in flight before it captures a new block.
+constexpr int kCaptureDepth = 2;
+
+// True when group k's ring slot for its next generation holds a captured
+// block of B generations that matches them (parity and local search).
+static bool block_cached(adgpu_job_t j, int k, int B) {
+    const RunGroup& g = j->groups[k];
+    const int slot0 = (int)(g.enq % kRing);
+    return g.blk[slot0] && g.blk_parity[slot0] == g.parity &&
+           g.blk_method[slot0] == ls_method_for(j, j->generation + g.enq) && g.blk_len[slot0] == B;
+}
+
 static int enqueue_block(adgpu_job_t j, int k, const LigandSet* ls, const MapSet* ms, int B) {
     adgpu_context* c = j->ctx;
     RunGroup& g = j->groups[k];
@@ -1412,6 +1424,10 @@ int adgpu_job_run_generations(adgpu_job_t j, int64_t max_new, int64_t* out_gens,
                                  surely_continues(g.enq + B - 1);
                     for (int i = 1; block && i < B; ++i)
                         block = ls_method_for(j, j->generation + g.enq + i) == ls_method_for(j, j->generation + g.enq);
+                    // capturing and instantiating a block takes the host a few milliseconds:
+                    // only while the group has generations in flight to hide them (a
+                    // block already captured for this ring slot launches at once)
+                    if (block && !block_cached(j, k, B) && g.enq - g.done < kCaptureDepth) block = false;
                     rc = block ? enqueue_block(j, k, ls, ms, B) : enqueue_generation(j, k, ls, ms);
                     if (rc) return rc;
                     moved = true;
