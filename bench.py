@@ -397,7 +397,11 @@ def _run_served(ctx, slots, reqs, mids, sp, ep, qps=0.0, seed=0):
                 if delay > 0:
                     time.sleep(delay)
             tickets.append(srv.submit(mids[m], tables, sp, init, seeds, ep=ep))
-        res = [srv.result(t) for t in tickets]
+        res = []
+        for k, t in enumerate(tickets):
+            res.append(srv.result(t))
+            if (k + 1) % 1000 == 0:   # progress of long runs (10,000-ligand serving)
+                print("  %d results, %.1f s" % (k + 1, time.perf_counter() - t0), file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         cpu1 = os.times()
         st = srv.stats()
