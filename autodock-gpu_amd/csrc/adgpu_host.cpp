@@ -812,13 +812,16 @@ static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire, bool
 }
 
 // Regrouping interval of the automatic setting, used when a job runs in four
-// or more run groups: their launches overlap, so the lanes that regrouping
-// frees in one launch's last searches run another launch's work.  Measured on
-// the 30-atom ligand, ADADELTA nrun 100 x pop 150 (M evals/s, interval 8 / 32 /
-// off): 4 groups 736 / 726 / 711; 2 groups 469 / 685 / 676; 1 group 247 / 524 /
-// 575.  With fewer groups the last searches of a launch set its length, and
-// each hand-over attempt (a few device-scope memory round trips on one
-// counter that every lone search uses) lengthens them.
+// or more run groups or in a server of four or more slots: the launches
+// overlap, so the lanes that regrouping frees in one launch's last searches
+// run another launch's work.  Measured on the 30-atom ligand, ADADELTA nrun
+// 100 x pop 150 (M evals/s, interval 8 / 32 / off): 4 groups 736 / 726 / 711;
+// 2 groups 469 / 685 / 676; 1 group 247 / 524 / 575.  Served, 32 slots x 128
+// requests of nrun 20 (k ligands/h, 4 repetitions): 112-115 / 109-114 /
+// 102-108.  With fewer groups and no other requests the last searches of a
+// launch set its length, and each hand-over attempt (a few device-scope
+// memory round trips on one counter that every lone search uses) lengthens
+// them.
 constexpr int kRegroupAuto = 8;
 
 // True when the job's ADADELTA searches hand off between teams.
@@ -837,7 +840,7 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
     if (yield_wanted(c, sp) && j->cap_yield >= (size_t)sp->num_runs * std::max(1, j->sd.num_ls)) {
         j->sd.ls_yield = c->ls_yield.load();
         const int rg = c->ls_regroup.load();
-        j->sd.ls_regroup = rg >= 0 ? rg : (j->groups.size() >= 4 ? kRegroupAuto : 0);
+        j->sd.ls_regroup = rg >= 0 ? rg : (j->groups.size() >= 4 || j->sharers >= 4 ? kRegroupAuto : 0);
     }
     j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
     const int nrun = sp->num_runs;

@@ -198,8 +198,8 @@ int  adgpu_set_ls_handoff(adgpu_ctx ctx, int iterations);
  * beside idle lanes merge and the freed lanes run other launches' work.
  * Bit-identical results, like the hand-off; 0 turns it off; -1 (the default,
  * also environment variable ADGPU_LS_REGROUP at adgpu_open) regroups every 8
- * iterations in jobs of four or more run groups, whose launches overlap, and
- * not otherwise.  Applies to jobs created afterwards.  No counterpart in the
+ * iterations in jobs of four or more run groups and in servers of four or
+ * more slots, whose launches overlap, and not otherwise.  Applies to jobs created afterwards.  No counterpart in the
  * reference (one search per block, cuda/kernel_ad.cu:60-120). */
 int  adgpu_set_ls_regroup(adgpu_ctx ctx, int iterations);
 
