@@ -105,11 +105,12 @@ DEV int seat_swap(int32_t* e, int v) {
 // queue and the ring are drained, a team whose wavefront still runs another
 // team's search, but has none of its own, takes a seat (a ticket Q[5] and a
 // seat entry) and looks at it once per iteration; a team whose search is the
-// only work left in its wavefront, after running it for ls_regroup iterations
-// there, saves it and hands it to the oldest free seat (a ticket Q[6] < Q[5])
-// if that seat is older than every seat its own wavefront holds: two
-// half-idle wavefronts become one busy and one finished, and the lanes the
-// idle teams spent on evaluations nobody used go to other launches.  A seated
+// only work left in its wavefront tries, whenever the search's iteration count
+// is a multiple of ls_regroup, to save it and hand it to the oldest free seat
+// (a ticket Q[6] < Q[5]) if that seat is older than every seat its own
+// wavefront holds: two half-idle wavefronts become one busy and one finished,
+// and the lanes the idle teams spent on evaluations nobody used go to other
+// launches.  A search moves at most once.  A seated
 // team whose wavefront has no other search left gives its seat up; giving up
 // and handing over race on the seat entry (one compare-and-swap each), so a
 // search handed to a seat is taken exactly once, and a hand-over that loses
