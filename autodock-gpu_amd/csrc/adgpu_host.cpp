@@ -823,6 +823,9 @@ static int job_reserve(adgpu_job* j, int nrun, int pop, int nls, bool fire, bool
 // memory round trips on one counter that every lone search uses) lengthens
 // them.
 constexpr int kRegroupAuto = 8;
+// ... and with two or three run groups (2 groups, M evals/s, interval 32 /
+// off, three dockings each: 680 / 686 / 690 against 670 / 678 / 681).
+constexpr int kRegroupAutoFew = 32;
 
 // True when the job's ADADELTA searches hand off between teams.
 static bool yield_wanted(const adgpu_context* c, const adgpu_search_params* sp) {
@@ -840,7 +843,9 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
     if (yield_wanted(c, sp) && j->cap_yield >= (size_t)sp->num_runs * std::max(1, j->sd.num_ls)) {
         j->sd.ls_yield = c->ls_yield.load();
         const int rg = c->ls_regroup.load();
-        j->sd.ls_regroup = rg >= 0 ? rg : (j->groups.size() >= 4 || j->sharers >= 4 ? kRegroupAuto : 0);
+        j->sd.ls_regroup = rg >= 0 ? rg
+                         : j->groups.size() >= 4 || j->sharers >= 4 ? kRegroupAuto
+                         : j->groups.size() >= 2 ? kRegroupAutoFew : 0;
     }
     j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
     const int nrun = sp->num_runs;

@@ -199,7 +199,8 @@ int  adgpu_set_ls_handoff(adgpu_ctx ctx, int iterations);
  * Bit-identical results, like the hand-off; 0 turns it off; -1 (the default,
  * also environment variable ADGPU_LS_REGROUP at adgpu_open) regroups every 8
  * iterations in jobs of four or more run groups and in servers of four or
- * more slots, whose launches overlap, and not otherwise.  Applies to jobs created afterwards.  No counterpart in the
+ * more slots, whose launches overlap, every 32 with two or three run groups,
+ * and not with one.  Applies to jobs created afterwards.  No counterpart in the
  * reference (one search per block, cuda/kernel_ad.cu:60-120). */
 int  adgpu_set_ls_regroup(adgpu_ctx ctx, int iterations);
 
