@@ -61,11 +61,12 @@ TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 483.3}
 # Wavefront instructions of the local-search kernel per local-search
 # evaluation (SQ_INSTS_VALU and SQ_INSTS_LDS summed over its dispatches, over
 # the LS evaluations of those dispatches), from a rocprofv3 --pmc pass over
-# this workload with one run group: profiles/k_ls_ad_c1_counters.md.  With the
-# measured evaluation rate they give the kernel's VALU issue rate, reported
-# against the device's VALU issue rate on independent FP32 FMA chains measured
-# live (adgpu_measure_valu_rate).  Keyed by (maps, nrun, pop, ls).
-LS_INSTR_PER_EVAL = {("f32", 100, 150, "ad"): (995.2, 83.6)}
+# this workload in its default four run groups (with regrouping):
+# profiles/k_ls_ad_c1_counters.md.  With the measured evaluation rate they give
+# the kernel's VALU issue rate, reported against the device's VALU issue rate
+# on independent FP32 FMA chains measured live (adgpu_measure_valu_rate).
+# Keyed by (maps, nrun, pop, ls).
+LS_INSTR_PER_EVAL = {("f32", 100, 150, "ad"): (892.9, 73.2)}
 
 
 def parse():
@@ -756,10 +757,10 @@ def main():
         out["roofline"]["valu"] = {"achieved": rate, "peak": peak, "unit": "wave-instr/s", "frac": rate / peak,
                                    "valu_instr_per_eval": valu_per_eval, "lds_instr_per_eval": lds_per_eval,
                                    "per_cu_per_clock_at_2p4ghz": rate / (256 * 2.4e9),
-                                   "source": "instructions per evaluation from dispatch counters with one run "
-                                             "group (dispatches serialized); peak = FP32 FMA chains measured live "
-                                             "by adgpu_measure_valu_rate; frac above 1 means the overlapping "
-                                             "launches issue this mix faster than the FMA loop"}
+                                   "source": "instructions per evaluation from dispatch counters, four run "
+                                             "groups with regrouping (dispatches serialized); peak = FP32 FMA "
+                                             "chains measured live by adgpu_measure_valu_rate; frac above 1 means "
+                                             "the overlapping launches issue this mix faster than the FMA loop"}
         out["roofline"]["note"] = "the launch is VALU-issue-bound (valu); HBM traffic is 0.17 of the " \
                                   "algorithmic bytes (the map set stays in L2 and the Infinity Cache)"
     if not a.no_cpu_baseline:
