@@ -101,6 +101,9 @@ def parse():
     ap.add_argument("--team-table", default="",
                     help="serve/screen: fixed team widths per ligand size, e.g. 32:32,128:64 (max atoms:width)")
     ap.add_argument("--serve-ligands", type=int, default=300)
+    ap.add_argument("--serve-distinct", type=int, default=0,
+                    help="serve: distinct ligands to generate (0: one per request); requests beyond them "
+                         "reuse the ligands in turn with their own seeds and initial populations")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-evals", type=int, default=1000000)
     ap.add_argument("--serve-nrun", type=int, default=20)
@@ -367,7 +370,12 @@ def _prepare(make, n, map_points, nrun, pop):
     from adgpu import synth
     t_prep = time.perf_counter()
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
-        ligs = pool.map(make, range(n))
+        ligs = []
+        for lig in pool.imap(make, range(n), chunksize=4):
+            ligs.append(lig)
+            if len(ligs) % 250 == 0:   # progress of long preparations
+                print("  %d ligands made, %.1f s" % (len(ligs), time.perf_counter() - t_prep), file=sys.stderr,
+                      flush=True)
     types = sorted({t for lig in ligs for t in lig.type_names})
     map_sets = [synth.make_maps(types, npts=p, spacing=0.375, seed=70 + p) for p in map_points]
     reqs = []
@@ -493,7 +501,12 @@ def _autotune(ctx, a, ls):
 def serve(a):
     """The serving-loop measurement (module docstring)."""
     from adgpu._abi import energy_params
-    map_sets, reqs = _prepare(_lognormal_ligand, a.serve_ligands, (60, 80, 100), a.serve_nrun, a.pop)
+    distinct = a.serve_distinct if 0 < a.serve_distinct < a.serve_ligands else a.serve_ligands
+    map_sets, reqs = _prepare(_lognormal_ligand, distinct, (60, 80, 100), a.serve_nrun, a.pop)
+    for k in range(distinct, a.serve_ligands):   # the same ligands again, own seeds and initial populations
+        m, tables, _, _ = reqs[k % distinct]
+        init, seeds = initial_state(5000 + k, a.serve_nrun, a.pop, map_sets[m])
+        reqs.append((m, tables, init, seeds))
     ep = energy_params()
     # the workers wait for the device in the kernel, not by spinning on host cores
     os.environ.setdefault("ADGPU_HOST_WAIT", "block")
@@ -514,9 +527,10 @@ def serve(a):
         "higher_is_better": True,
         "dtype": "f32 (FP16 map storage)",
         "data": "synthetic (seeded ligands and maps generated in-process)",
-        "config": {"workload": "serving loop: %d ligands (atoms lognormal median 35, max 90; torsions 0-25) on "
+        "config": {"workload": "serving loop: %d ligands%s (atoms lognormal median 35, max 90; torsions 0-25) on "
                                "three resident FP16 map sets (60^3, 80^3, 100^3), FIRE, nrun %d x %d evals, pop %d"
-                               % (len(reqs), a.serve_nrun, a.serve_evals, a.pop),
+                               % (len(reqs), (" (%d distinct)" % distinct) if distinct < len(reqs) else "",
+                                  a.serve_nrun, a.serve_evals, a.pop),
                    "slots": a.serve_slots, "hw_queues": a.hw_queues,
                    "host_wait": os.environ.get("ADGPU_HOST_WAIT")},
         "team_table": tuned,
