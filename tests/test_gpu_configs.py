@@ -115,8 +115,8 @@ def test_screen_fp16_maps_give_the_fp32_median_best_energy(gpu_ctx):
     and once with FP16 map storage.  Each run's best pose is re-scored in FP64
     on the FP32 maps; a ligand's best energy is the lowest over its runs.  The
     median over the ligands of the best energy with FP16 maps equals the one
-    with FP32 maps within 0.1 kcal/mol (the north star's "equal median best
-    energy"), and so does the mean paired difference."""
+    with FP32 maps within 0.1 kcal/mol (FP16 storage at equal median best
+    energy), and so does the mean paired difference."""
     import time
     ctx = gpu_ctx
     rng = np.random.default_rng(4711)
