@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 # copy kernel (FETCH_SIZE reports 0.5 GiB for the 1 GiB read: factor 2.0;
 # WRITE_SIZE reports 1 GiB: factor 1.0).  Summary and commands:
 # profiles/k_ls_ad_c1_traffic.md.  Keyed by (maps, nrun, pop, ls).
-TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 483.3}
+TRAFFIC_BYTES_PER_LS_EVAL = {("f32", 100, 150, "ad"): 494.3}
 
 # Wavefront instructions of the local-search kernel per local-search
 # evaluation (SQ_INSTS_VALU and SQ_INSTS_LDS summed over its dispatches, over
