@@ -82,9 +82,11 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
     try:
         ref = _dock(gpu_ctx, mid, lid, sp, init, seeds, 0)
         assert ref[3] == 0 and ref[4]["regroups"] == 0
+        steps = {}
         for h, rg in ((1, 8), (32, 0), (32, 8), (32, 1)):
             got = _dock(gpu_ctx, mid, lid, sp, init, seeds, h, rg)
             st = got[4]
+            steps[(h, rg)] = st["wave_steps"]
             print("atoms", atoms, "width", width, "hand-off", h, "regroup", rg, "searches handed off", got[3],
                   "regrouped", st["regroups"], "wavefront steps", st["wave_steps"], flush=True)
             # the hand-off took place (the bit-identity below is not vacuous)
@@ -99,6 +101,10 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
             assert got[0] == ref[0] and got[1] == ref[1], (h, rg, got[:2], ref[:2])
             for k in ref[2]:
                 assert np.array_equal(got[2][k], ref[2][k]), (h, rg, k)
+        tw = width if width else (64 if atoms > 64 else 32)
+        if tw < 64:
+            # the same evaluations in fewer wavefront steps: fewer idle teams beside busy ones
+            assert steps[(32, 8)] < steps[(32, 0)], steps
     finally:
         gpu_ctx.set_ls_handoff(32)
         gpu_ctx.set_ls_regroup(-1)
