@@ -113,6 +113,35 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
         gpu_ctx.maps_release(mid)
 
 
+@pytest.mark.parametrize("width", [16, 32])
+def test_regroup_with_few_searches(gpu_ctx, width):
+    """Few searches per launch (8 runs x 3 of pop 50, lsearch_rate 6 %) against
+    thousands of teams: most teams find nothing from the start and the seat
+    tickets (twice the searches) run out; every regrouping interval still
+    gives the same docking as none."""
+    lig = _ligand(24, 5, 321)
+    maps = synth.make_maps(lig.type_names, npts=48, spacing=0.375, seed=24)
+    tables = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+    mid = gpu_ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing)
+    lid = gpu_ctx.ligand_upload(tables, energy_params())
+    sp = search_params(num_runs=8, pop_size=50, max_evals=60000, ls="ad", spacing=maps.spacing, lsearch_rate=6.0)
+    init, seeds = _initial(8, 50, maps, 24)
+    gpu_ctx.set_team_width(width)
+    try:
+        ref = _dock(gpu_ctx, mid, lid, sp, init, seeds, 32, 0)
+        for rg in (1, 8):
+            got = _dock(gpu_ctx, mid, lid, sp, init, seeds, 32, rg)
+            print("few searches: width", width, "regroup", rg, got[4], flush=True)
+            assert got[0] == ref[0] and got[1] == ref[1], (rg, got[:2], ref[:2])
+            for k in ref[2]:
+                assert np.array_equal(got[2][k], ref[2][k]), (rg, k)
+    finally:
+        gpu_ctx.set_ls_regroup(-1)
+        gpu_ctx.set_team_width(0)
+        gpu_ctx.ligand_release(lid)
+        gpu_ctx.maps_release(mid)
+
+
 def test_handoff_argument(gpu_ctx):
     with pytest.raises(adgpu.AdgpuError):
         gpu_ctx.set_ls_handoff(-1)
