@@ -28,6 +28,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_adam(Launch a) {
     int iters = 0;
     size_t gi = 0;
     bool done = false, need = true;
+    uint32_t steps = 0;              // wavefront evaluation steps (tallied once per wavefront)
     for (;;) {
         if (need) {   // uniform within the team
             int t = 0;
@@ -51,7 +52,11 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_adam(Launch a) {
                 best_e = INFINITY; iters = 0;
             }
         }
-        if (!TM::any_in_wave(!done)) break;
+        if (!TM::any_in_wave(!done)) {
+            if (lane_id() == 0 && steps != 0u) atomicAdd(&a.B.ls_counter[8], (int)steps);
+            break;
+        }
+        ++steps;
         float g[GPL];
         const float e = evaluate<TW, NC, MF, true>(X, geno, g);
         if (!done) {

@@ -58,6 +58,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_fire(Launch a) {
     int count_success = 0, iters = 0;
     size_t gi = 0;
     bool done = false, need = true;
+    uint32_t steps = 0;              // wavefront evaluation steps (tallied once per wavefront)
     for (;;) {
         if (need) {   // uniform within the team
             int t = 0;
@@ -90,7 +91,11 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_fire(Launch a) {
 #pragma unroll
             for (int j = 0; j < GPL; ++j) vel[j] = done ? 0.f : -grad[j] * inv_gn;
         }
-        if (!TM::any_in_wave(!done)) break;
+        if (!TM::any_in_wave(!done)) {
+            if (lane_id() == 0 && steps != 0u) atomicAdd(&a.B.ls_counter[8], (int)steps);
+            break;
+        }
+        ++steps;
         float cand[GPL], cg[GPL];
 #pragma unroll
         for (int j = 0; j < GPL; ++j) cand[j] = geno[j] + dt * vel[j];

@@ -41,6 +41,7 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_sw(Launch a) {
     int cons_succ = 0, cons_fail = 0, iters = 0, evals = 0, phase = 0, t = 0;
     size_t gi = 0;
     bool done = false, need = true;
+    uint32_t steps = 0;              // wavefront evaluation steps (tallied once per wavefront)
     for (;;) {
         while (need) {   // uniform within the team
             int tt = 0;
@@ -78,7 +79,11 @@ __global__ ADGPU_GROUP_KERNEL void k_ls_sw(Launch a) {
                 if (tl == 0) a.B.ene_next[gi] = off_e;
             }
         }
-        if (!TM::any_in_wave(!done)) break;
+        if (!TM::any_in_wave(!done)) {
+            if (lane_id() == 0 && steps != 0u) atomicAdd(&a.B.ls_counter[8], (int)steps);
+            break;
+        }
+        ++steps;
         if (!done && phase == 0) {
 #pragma unroll
             for (int j = 0; j < GPL; ++j) {

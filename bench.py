@@ -750,12 +750,13 @@ def main():
         "generations_per_ligand": float(np.mean(gens_list)),
         "kernels": {"ls_avg_ms": avg_ls_ms, "ls_busy_ms_per_launch": eff_ls_ms, "gen_avg_ms": float(np.mean(gen_ms)),
                     "ls_launches": int(np.sum(ls_n)), "ls_busy_share_of_wall": float(np.sum(ls_busy)) / 1000.0 / t,
-                    # ADADELTA: searches handed off and regrouped per docking, and the share
-                    # of team evaluation steps idle teams spent beside busy ones
+                    # ADADELTA: searches handed off and regrouped per docking; every local
+                    # search: the share of team evaluation steps idle teams spent beside
+                    # busy ones
                     "ls_handoffs": float(np.mean([x["handoffs"] for x in ls_stats_list])),
                     "ls_regroups": float(np.mean([x["regroups"] for x in ls_stats_list])),
                     "idle_team_share": (1.0 - ls_evals / (sum(x["wave_steps"] for x in ls_stats_list) * 64.0 / tw_used)
-                                        if a.ls == "ad" and sum(x["wave_steps"] for x in ls_stats_list) else None)},
+                                        if sum(x["wave_steps"] for x in ls_stats_list) else None)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "duration": "LS busy time (union of launch intervals) / launches",

@@ -311,7 +311,7 @@ int  adgpu_job_kernel_busy(adgpu_job_t job, int kernel_class, double* busy_ms, i
 int  adgpu_job_ls_handoffs(adgpu_job_t job, int64_t* count);
 /* Local-search tallies since the job started: stats[0] hand-offs (as
  * adgpu_job_ls_handoffs), stats[1] regroups (searches moved to a seat,
- * adgpu_set_ls_regroup), stats[2] wavefront evaluation steps of the ADADELTA
+ * adgpu_set_ls_regroup), stats[2] wavefront evaluation steps of the local-search
  * launches (each counts one evaluation for every team of the wavefront, idle
  * teams included, modulo 2^32 per run group), stats[3] 0.  With the job's
  * local-search evaluations E and teams of TW lanes, 1 - E / (stats[2] * 64 / TW)
