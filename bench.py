@@ -774,8 +774,9 @@ def main():
                                    "per_cu_per_clock_at_2p4ghz": rate / (256 * 2.4e9),
                                    "source": "instructions per evaluation from dispatch counters, four run "
                                              "groups with regrouping (dispatches serialized); peak = FP32 FMA "
-                                             "chains measured live by adgpu_measure_valu_rate; frac above 1 means "
-                                             "the overlapping launches issue this mix faster than the FMA loop"}
+                                             "chains measured live by adgpu_measure_valu_rate; frac above 1: the "
+                                             "overlapping launches issue this mix faster than the FMA loop (its "
+                                             "v_mov and DPP moves take a fraction of an FMA's issue slot)"}
         out["roofline"]["note"] = "the launch is VALU-issue-bound (valu); HBM traffic is 0.17 of the " \
                                   "algorithmic bytes (the map set stays in L2 and the Infinity Cache)"
     if not a.no_cpu_baseline:
