@@ -255,11 +255,11 @@ class Job:
         return n.value
 
     def ls_stats(self):
-        """{"handoffs", "regroups", "wave_steps"} since the job started
-        (adgpu_job_ls_stats)."""
+        """{"handoffs", "regroups", "wave_steps"} since the job started, and
+        the job's "team_width" (adgpu_job_ls_stats)."""
         a = (C.c_int64 * 4)()
         self._c(self.lib.adgpu_job_ls_stats(self.h, a), "adgpu_job_ls_stats")
-        return {"handoffs": a[0], "regroups": a[1], "wave_steps": a[2]}
+        return {"handoffs": a[0], "regroups": a[1], "wave_steps": a[2], "team_width": a[3]}
 
     def set_autostop(self, enable=True, stopstd=0.15, as_frequency=5, min_as_evals=0):
         """AutoStop tested on the device every as_frequency generations."""

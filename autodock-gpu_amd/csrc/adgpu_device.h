@@ -1009,9 +1009,7 @@ inline Shape choose_shape(const LigandDev& L, int team_width_hint, int threads =
     Shape s{};
     s.threads = threads;
     const int N = L.num_atoms;
-    int tw = team_width_hint == 64 ? 64 : (team_width_hint == 32 ? 32 : (N <= 128 ? 32 : 64));
-    if (team_width_hint == 16 && N <= 32) tw = 16;
-    if (tw == 32 && N > 128) tw = 64;
+    const int tw = team_lanes_for(N, team_width_hint);
     s.tw = tw;
     const int per = (N + tw - 1) / tw;
     s.nc = per <= 1 ? 1 : (per <= 2 ? 2 : 4);

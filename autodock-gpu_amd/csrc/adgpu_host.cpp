@@ -112,6 +112,7 @@ struct adgpu_job {
     int want_groups = 1;                // run groups of a serving slot (fewer when a request has fewer runs)
     int sharers = 1;                    // requests that may run at once with this job (a server's slots)
     int team_hint = 0;                  // team width of the job's kernels, fixed when it starts
+    int team_lanes = 0;                 // the lanes per team that width gives the job's ligand
     int graph_block = 0;                // generations per captured graph, fixed when it starts (0: none)
     LigandSet own{};
     size_t own_cap = 0;
@@ -191,7 +192,7 @@ LaunchArgs launch_args(adgpu_context* c, const LigandSet& ls, const MapSet& ms, 
 constexpr int64_t kWideTeamSearches = 20000;
 
 // A job's team width, fixed when it starts: the context's width or table,
-// else 32 lanes up to 32 atoms and 64 above 64; between, 64 lanes when the
+// else (up to 32 atoms, see below) 32 lanes, and 64 above 64; between, 64 lanes when the
 // searches in flight -- the job's own times the requests that may share the
 // device with it (a server's slots) -- are below kWideTeamSearches, else 32.
 // Above 64 atoms a 32-lane team carries three or four atom chunks per lane;
@@ -205,12 +206,32 @@ constexpr int64_t kWideTeamSearches = 20000;
 // 176 M evals/s.  Shared: 4 slots x ADAM nrun 50 (30000 searches), 10-60
 // atoms, 16.0 vs 14.5 k ligands/h; 32 slots x FIRE nrun 20 (96000 searches),
 // 85.4 vs 82.8 k ligands/h (64 from 33 atoms).
-int job_team_width(adgpu_context* c, int num_atoms, int64_t searches, int sharers) {
+//
+// Up to 32 atoms the local search decides: Solis-Wets and ADADELTA searches
+// end after uneven iteration counts, so the two searches of a 32-lane
+// wavefront seldom end together, and while the searches are few the
+// half-idle wavefronts are a large share of each launch; 64-lane teams have
+// none, at more instructions per evaluation.  Measured on the 30-atom ligand
+// alone (2.5 M evaluations per run), M evals/s, 32 vs 64 lanes: Solis-Wets
+// nrun 10 / 20 / 30 / 40 / 45 / 50: 232 / 448 / 631 / 781 / 865 / 941 vs
+// 290 / 539 / 748 / 846 / 887 / 924; ADADELTA nrun 20 / 30 / 40 / 50 / 60 /
+// 80 / 100: 211 / 288 / 356 / 431 / 502 / 646 / 740 vs 227 / 324 / 452 / 518
+// / 533 / 538 / 571.  FIRE and ADAM run fixed iteration counts and keep 32
+// lanes (nrun 20: FIRE 436 vs 423, ADAM 414 vs 380).
+constexpr int64_t kWideTeamSwSearches = 7000;
+constexpr int64_t kWideTeamAdSearches = 10000;
+
+int job_team_width(adgpu_context* c, int num_atoms, int64_t searches, int sharers, int ls_method) {
     const int w = context_team_width(c, num_atoms);
     if (w) return w;
-    if (num_atoms <= 32) return 32;
+    const int64_t in_flight = searches * std::max(1, sharers);
+    if (num_atoms <= 32) {
+        const int64_t few = ls_method == ADGPU_LS_SW ? kWideTeamSwSearches
+                          : ls_method == ADGPU_LS_AD ? kWideTeamAdSearches : 0;
+        return in_flight < few ? 64 : 32;
+    }
     if (num_atoms > 64) return 64;
-    return searches * std::max(1, sharers) < kWideTeamSearches ? 64 : 32;
+    return in_flight < kWideTeamSearches ? 64 : 32;
 }
 
 // Message of the calling thread's last failed call (jobs of one context may
@@ -847,7 +868,9 @@ static int job_start(adgpu_job* j, const adgpu_search_params* sp, const LigandDe
                          : j->groups.size() >= 4 || j->sharers >= 4 ? kRegroupAuto
                          : j->groups.size() >= 2 ? kRegroupAutoFew : 0;
     }
-    j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers);
+    j->team_hint = job_team_width(c, lig.num_atoms, (int64_t)sp->num_runs * std::max(1, j->sd.num_ls), j->sharers,
+                                  sp->ls_method);
+    j->team_lanes = team_lanes_for(lig.num_atoms, j->team_hint);
     const int nrun = sp->num_runs;
     const size_t ne = (size_t)nrun * sp->pop_size;
     j->graph_block = c->graph_block.load();
@@ -1102,7 +1125,7 @@ int adgpu_job_ls_stats(adgpu_job_t j, int64_t* stats) {
         r += q[16 * k + 7];
         w += (uint32_t)q[16 * k + 8];
     }
-    stats[0] = h; stats[1] = r; stats[2] = w; stats[3] = 0;
+    stats[0] = h; stats[1] = r; stats[2] = w; stats[3] = j->team_lanes;
     return ADGPU_OK;
 }
 

@@ -74,6 +74,15 @@ struct SearchDev {
 // E[g^2], E[dx^2] (kGenes floats each), then best energy, varrho,
 // consecutive successes, consecutive failures, iterations and the
 // population index (integers stored as their bit patterns).
+// Lanes per team for a ligand of n atoms and a width hint (16, 32, 64, or
+// another value for the default): 16 only up to 32 atoms, 32 only up to 128.
+inline int team_lanes_for(int n_atoms, int hint) {
+    int tw = hint == 64 ? 64 : (hint == 32 ? 32 : (n_atoms <= 128 ? 32 : 64));
+    if (hint == 16 && n_atoms <= 32) tw = 16;
+    if (tw == 32 && n_atoms > 128) tw = 64;
+    return tw;
+}
+
 constexpr int kAdStateFloats = 4 * kGenes + 8;
 
 struct JobBuffers {

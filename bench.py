@@ -670,8 +670,6 @@ def main():
 
     lat, evals_list, gens_list, ls_ms, ls_n, gen_ms, ls_busy = [], [], [], [], [], [], []
     ls_evals_list, ls_stats_list = [], []
-    # lanes per team: the library runs ligands of at most 32 atoms in 32-lane teams
-    tw_used = a.team_width if a.team_width else 32
     for step in range(a.warmup + a.steps):
         init, seeds = initial_state(1000 + step, a.nrun, a.pop, maps)
         job = ctx.job(mid, lid, sp, init, seeds)      # allocations outside the timed region
@@ -755,8 +753,10 @@ def main():
                     # busy ones
                     "ls_handoffs": float(np.mean([x["handoffs"] for x in ls_stats_list])),
                     "ls_regroups": float(np.mean([x["regroups"] for x in ls_stats_list])),
-                    "idle_team_share": (1.0 - ls_evals / (sum(x["wave_steps"] for x in ls_stats_list) * 64.0 / tw_used)
-                                        if sum(x["wave_steps"] for x in ls_stats_list) else None)},
+                    "idle_team_share": (1.0 - ls_evals / sum(x["wave_steps"] * 64.0 / x["team_width"]
+                                                             for x in ls_stats_list)
+                                        if sum(x["wave_steps"] for x in ls_stats_list) else None),
+                    "team_width": int(ls_stats_list[0]["team_width"]) if ls_stats_list else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "duration": "LS busy time (union of launch intervals) / launches",

@@ -313,9 +313,10 @@ int  adgpu_job_ls_handoffs(adgpu_job_t job, int64_t* count);
  * adgpu_job_ls_handoffs), stats[1] regroups (searches moved to a seat,
  * adgpu_set_ls_regroup), stats[2] wavefront evaluation steps of the local-search
  * launches (each counts one evaluation for every team of the wavefront, idle
- * teams included, modulo 2^32 per run group), stats[3] 0.  With the job's
- * local-search evaluations E and teams of TW lanes, 1 - E / (stats[2] * 64 / TW)
- * is the share of team evaluation steps that idle teams spent.  No counterpart
+ * teams included, modulo 2^32 per run group), stats[3] the job's team width
+ * TW (lanes per team).  With the job's local-search evaluations E,
+ * 1 - E / (stats[2] * 64 / TW) is the share of team evaluation steps that
+ * idle teams spent.  No counterpart
  * in the reference: measurement. */
 int  adgpu_job_ls_stats(adgpu_job_t job, int64_t stats[4]);
 

@@ -94,6 +94,7 @@ def test_handoff_is_bit_identical(gpu_ctx, atoms, tors, width):
             # searches regrouped, except with one team per wavefront (nothing to merge)
             # default widths of these jobs: 32 lanes at 30 atoms, 64 above 64 atoms
             tw = width if width else (64 if atoms > 64 else 32)
+            assert st["team_width"] == tw, st
             if rg > 0 and tw < 64:
                 assert st["regroups"] > 0, st
             else:
@@ -137,6 +138,39 @@ def test_regroup_with_few_searches(gpu_ctx, width):
                 assert np.array_equal(got[2][k], ref[2][k]), (rg, k)
     finally:
         gpu_ctx.set_ls_regroup(-1)
+        gpu_ctx.set_team_width(0)
+        gpu_ctx.ligand_release(lid)
+        gpu_ctx.maps_release(mid)
+
+
+@pytest.mark.parametrize("ls,nrun,want", [("sw", 20, 64), ("sw", 50, 32), ("ad", 20, 64), ("ad", 60, 64),
+                                          ("ad", 100, 32), ("fire", 20, 32), ("adam", 20, 32)])
+def test_default_team_width_for_small_ligands(gpu_ctx, ls, nrun, want):
+    """Up to 32 atoms the default width follows the local search: Solis-Wets
+    below 7000 and ADADELTA below 10000 searches in flight per generation run
+    in 64-lane teams (their searches end unevenly, and few searches leave
+    many half-idle wavefronts), FIRE and ADAM, and more searches, in 32-lane
+    teams.  adgpu_job_ls_stats reports the width."""
+    lig = _ligand(30, 6, 130)
+    maps = synth.make_maps(lig.type_names, npts=40, spacing=0.375, seed=30)
+    tables = synth.ligand_tables(lig, maps.spacing, maps.type_names)
+    mid = gpu_ctx.maps_upload(maps.packed(), maps.size, maps.num_map_atypes, maps.spacing)
+    lid = gpu_ctx.ligand_upload(tables, energy_params())
+    try:
+        sp = search_params(num_runs=nrun, pop_size=POP, max_evals=20000, ls=ls, spacing=maps.spacing)
+        init, seeds = _initial(nrun, POP, maps, 30)
+        job = gpu_ctx.job(mid, lid, sp, init, seeds)
+        try:
+            assert job.ls_stats()["team_width"] == want
+        finally:
+            job.destroy()
+        gpu_ctx.set_team_width(32)   # a fixed width wins
+        job = gpu_ctx.job(mid, lid, sp, init, seeds)
+        try:
+            assert job.ls_stats()["team_width"] == 32
+        finally:
+            job.destroy()
+    finally:
         gpu_ctx.set_team_width(0)
         gpu_ctx.ligand_release(lid)
         gpu_ctx.maps_release(mid)

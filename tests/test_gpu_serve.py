@@ -245,9 +245,9 @@ def test_admission_bounds_the_device_memory_slots_hold(gpu_ctx, mix):
     hipMemGetInfo, while the requests run and after they finish, stays within
     the budget (plus 10 % and 8 MB for the allocator's rounding); without the
     bound, every slot that once ran a large request would keep its 18 MB.
-    Results are again the same as alone."""
-    import threading
-    import time
+    Results are again the same as alone, with the team width fixed: left to
+    the default, a server of one slot and one of eight may pick different
+    widths for the same request (the searches in flight differ)."""
     q = mix[0]
     big_sp = search_params(num_runs=64, pop_size=150, max_evals=20000, ls="ad", spacing=0.375)
     g = q["cg"].s
@@ -260,6 +260,28 @@ def test_admission_bounds_the_device_memory_slots_hold(gpu_ctx, mix):
             reqs.append(dict(q, sp=big_sp, init=big_init, seeds=big_seeds, autostop=None, k="big%d" % k))
         else:
             reqs.append(dict(mix[k % len(mix)], k="small%d" % k))
+    gpu_ctx.set_team_width(32)
+    try:
+        ref, served, stats, peak, held, budget, big = _bounded_run(gpu_ctx, reqs)
+    finally:
+        gpu_ctx.set_team_width(0)
+    print("budget", budget, "big", big, "peak", peak[0], "held after", held, "stats", stats, flush=True)
+    slack = 0.10 * budget + (8 << 20)
+    assert stats["peak_bytes"] <= budget and stats["bytes_in_use"] <= budget
+    assert peak[0] <= budget + slack, (peak[0], budget)
+    assert held <= budget + slack, (held, budget)
+    assert 6 * big > budget + slack        # what slots would keep without the bound
+    for r, a, s in zip(reqs, ref, served):
+        assert s["status"] == 0, s["message"]
+        _same(a, s, r["k"])
+
+
+def _bounded_run(gpu_ctx, reqs):
+    """The requests one at a time through one slot, then all at once through
+    eight slots under a budget of 2.5 large requests, sampling the device
+    memory in use."""
+    import threading
+    import time
     ref, _ = _serve(gpu_ctx, reqs, slots=1, one_at_a_time=True)      # also warms up the runtime
     big = max(r["device_bytes"] for r in ref)
     budget = int(2.5 * big)
@@ -282,12 +304,4 @@ def test_admission_bounds_the_device_memory_slots_hold(gpu_ctx, mix):
         th.join()
         held = base - _device_free_bytes()
         stats = srv.stats()
-    print("budget", budget, "big", big, "peak", peak[0], "held after", held, "stats", stats, flush=True)
-    slack = 0.10 * budget + (8 << 20)
-    assert stats["peak_bytes"] <= budget and stats["bytes_in_use"] <= budget
-    assert peak[0] <= budget + slack, (peak[0], budget)
-    assert held <= budget + slack, (held, budget)
-    assert 6 * big > budget + slack        # what slots would keep without the bound
-    for r, a, s in zip(reqs, ref, served):
-        assert s["status"] == 0, s["message"]
-        _same(a, s, r["k"])
+    return ref, served, stats, peak, held, budget, big
