@@ -163,9 +163,10 @@ const char* adgpu_last_error(adgpu_ctx ctx);
 /* Version string; also usable without a GPU. */
 const char* adgpu_version(void);
 /* Lanes that evaluate one individual: 0 = chosen per ligand (the team table
- * below if set; else for a job 32 up to 32 atoms, 64 above 128 atoms, and
- * between them 64 while the local searches in flight per generation -- the
- * job's own times a server's slots -- stay below 20000, else 32; for
+ * below if set; else for a job, with the local searches in flight per
+ * generation counted as the job's own times a server's slots: up to 32 atoms
+ * 64 for Solis-Wets below 7000 and ADADELTA below 10000 searches in flight,
+ * else 32; 33-64 atoms 64 below 20000, else 32; 64 above 64 atoms; for
  * per-pose calls 32 up to 128 atoms), 16 (four individuals per
  * wavefront, ligands of at most 32 atoms; wider teams otherwise), 32 (two
  * individuals per wavefront) or 64.  Plays the role of the reference's
